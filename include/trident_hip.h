@@ -1,0 +1,211 @@
+/*
+ * trident_hip.h — C-ABI of the MI355X (gfx950) local SpGEMM path.
+ *
+ * This is the drop-in boundary for the single-GPU share of Trident's
+ * SpGEMM: the local tile multiply, the bitmask tile compression and the
+ * stage-wise partial-C accumulation.  Plain C: device pointers, sizes and
+ * status codes, no C++ or HIP types in the signatures (the stream is passed
+ * as void*).  The C++ shim include/KokkosWrap.hpp rebuilds the reference's
+ * KokkosWrap::LocalMatrix interface on top of it.
+ *
+ * Matrices are int32 CSR/CSC with fp64 values (the reference instantiates
+ * <int32_t,float>, src/hns_spgemm.cu:1116-1118; fp64 is the precision this
+ * path is built for).  Outputs produced by the library are stored in the
+ * reference's contiguous tile layout (include/tile_holder.cuh:74-89):
+ *     buf = [ double val[nnz] | int32 idx[nnz] | int32 ptr[ptrdim+1] ]
+ * with contig = 1, buf/buf_size set, and are owned by the context until
+ * released with tr_csx_free.
+ *
+ * Semantics (the contract this path defines where the reference delegates
+ * to Kokkos-Kernels 4.7.01 / cuSPARSE):
+ *  - output columns are sorted ascending in every row;
+ *  - structural zeros are kept (an entry exists iff a product lands on it);
+ *  - C = A*B accumulates, for every output, the products in the order of
+ *    A's stored entries: c_ij = fma(a_ik, b_kj, c_ij) from +0.0, so results
+ *    are bit-identical run to run, alone or batched;
+ *  - inputs of spgemm/spadd need sorted columns within rows (validated; an
+ *    unsorted operand is rejected with TR_ERR_INVALID and tr_csx_sort_rows
+ *    sorts one in place).
+ *
+ * Every call is ordered on the context's stream.  Calls that return sizes
+ * (spgemm, spadd, compress, symbolic) synchronise that stream once to read
+ * them, like the reference's own host round trips.  A context must not be
+ * used by two host threads at once (the reference keeps one stream per
+ * thread, include/common.h:13).  Errors are returned, never exit().
+ */
+#ifndef TRIDENT_HIP_H
+#define TRIDENT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TR_OK            0
+#define TR_ERR_INVALID   1   /* bad arguments or malformed/unsorted CSR   */
+#define TR_ERR_OOM       2   /* device allocation failed                  */
+#define TR_ERR_OVERFLOW  3   /* result nnz exceeds INT32_MAX (int32 tile) */
+#define TR_ERR_HIP       4   /* a HIP runtime call failed                 */
+
+/* majordim: mirrors mmio::MajorDim (ROWS = CSR, COLS = CSC). */
+#define TR_ROWS 0
+#define TR_COLS 1
+
+/* Mirrors mmio::CSX (fields as used at include/KokkosWrap.hpp:95-104,
+ * include/tile_holder.cuh:215-226, include/utils.cuh:455-466).  All
+ * pointers are device pointers.  ptr_vec == NULL marks an uninitialised
+ * matrix (LocalMatrix::initialized == false, KokkosWrap.hpp:49,208). */
+typedef struct tr_csx {
+    int32_t  majordim;  /* TR_ROWS or TR_COLS                                  */
+    int32_t  nrows;
+    int32_t  ncols;
+    int32_t  contig;    /* 1 if val|idx|ptr live in one buffer `buf`            */
+    int64_t  nnz;
+    int32_t *ptr_vec;   /* (majordim == ROWS ? nrows : ncols) + 1 entries       */
+    int32_t *idx_vec;   /* nnz                                                  */
+    double  *val;       /* nnz                                                  */
+    void    *buf;       /* base of the contiguous buffer (NULL if not contig)   */
+    size_t   buf_size;  /* bytes of buf                                         */
+} tr_csx_t;
+
+typedef struct tr_ctx *tr_ctx_t;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Create a context on `device`.  hip_stream: an existing hipStream_t to order
+ * work on, or NULL to create a private default-priority stream. */
+int tr_ctx_create(tr_ctx_t *ctx, int device, void *hip_stream);
+int tr_ctx_destroy(tr_ctx_t ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char *tr_last_error(tr_ctx_t ctx);
+/* Block until all work queued on the context's stream has finished. */
+int tr_ctx_synchronize(tr_ctx_t ctx);
+/* The hipStream_t work is ordered on. */
+void *tr_ctx_stream(tr_ctx_t ctx);
+
+/* ---- local multiply and accumulation (KokkosWrap::LocalMatrix) ---------- */
+
+/* C = A*B.  Replaces LocalMatrix::spgemm (include/KokkosWrap.hpp:367-391).
+ * A, B: TR_ROWS.  C_out receives a context-owned contiguous tile.
+ * TR_ERR_OVERFLOW if nnz(C) > INT32_MAX (use tr_spgemm_panels). */
+int tr_spgemm(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *C_out);
+
+/* C = A*B split into row panels of at most max_panel_nnz entries each
+ * (<= INT32_MAX), for products too large for one int32 tile.  Panel p holds
+ * rows [row_begin[p], row_begin[p+1]) of C.  panels/row_begin have room for
+ * max_panels (+1) entries; *npanels receives the count.  No counterpart in
+ * the reference, whose int32 size_type cannot hold such a C
+ * (KokkosWrap.hpp:46); it is how a C bigger than a tile is returned as tiles. */
+int tr_spgemm_panels(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max_panel_nnz,
+                     tr_csx_t *panels, int32_t *row_begin, int32_t max_panels, int32_t *npanels);
+
+/* C += A*B; if C is uninitialised (ptr_vec == NULL) C := A*B.
+ * Replaces LocalMatrix::sp_mma (include/KokkosWrap.hpp:315-364) and
+ * sp_mma_hybrid (include/cusparse_helpers.cuh:708-739: an empty product
+ * leaves C untouched).  A context-owned C is released and replaced by the
+ * new accumulator; a caller-owned C is left intact and replaced. */
+int tr_sp_mma(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *C_inout);
+
+/* C = A + C.  Replaces LocalMatrix::spadd (include/KokkosWrap.hpp:394-439)
+ * and cusparse_spgeam (include/cusparse_helpers.cuh:799-905, alpha=beta=1).
+ * A.nnz == 0: no-op; C uninitialised: C := copy of A. */
+int tr_spadd(tr_ctx_t ctx, const tr_csx_t *A, tr_csx_t *C_inout);
+
+/* Exact nnz(C) and the device bytes tr_spgemm needs for C = A*B (output tile
+ * plus peak scratch): the admission bound used by the serving loop.  No
+ * counterpart in the reference (its buffers grow on demand,
+ * include/utils.cuh:671-716). */
+int tr_symbolic(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *nnz_C,
+                size_t *bytes_needed);
+
+/* Number of multiplications sum_k nnz(A[:,k]) * nnz(B[k,:]) (device pass). */
+int tr_spgemm_flops(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *flops);
+
+/* ---- bitmask tile compression (SpaComm) --------------------------------- */
+
+/* mask[w] bit b set iff segment 64*w+b of ptr is non-empty; ceil(n/64)
+ * uint64 words.  Replaces SpaComm::gen_bitmask (include/spacomm.cuh:29-117)
+ * with 64-bit words (MASK_SIZE=64); on little-endian hosts the words are the
+ * same bytes as the reference's MASK_SIZE=8 uint8 mask. */
+int tr_bitmask_gen(tr_ctx_t ctx, const int32_t *ptr, int32_t n, uint64_t *mask);
+
+/* out[w] = a[w] & b[w].  Replaces SpaComm::intersect_bitmasks
+ * (include/spacomm.cuh:126-148). */
+int tr_bitmask_and(tr_ctx_t ctx, const uint64_t *a, const uint64_t *b, int32_t words, uint64_t *out);
+
+/* Keep the entries of major segment i (row for TR_ROWS, column for TR_COLS)
+ * iff bit i of mask is set; dimensions and majordim unchanged, dropped
+ * segments become empty.  `out` receives a context-owned contiguous tile.
+ * Replaces SpaCommHandler::Compress (include/spacomm.cuh:962-1091). */
+int tr_compress(tr_ctx_t ctx, const tr_csx_t *M, const uint64_t *mask, tr_csx_t *out);
+
+/* ---- conversions --------------------------------------------------------- */
+
+/* Transpose: out = CSR of M^T (equivalently M re-expressed in the other
+ * major dimension), context-owned, rows sorted.  Replaces
+ * LocalMatrix::csc_to_csr / cusparse_csc_to_csr (KokkosWrap.hpp:441-531,
+ * cusparse_helpers.cuh:607-705) when applied to a CSC tile. */
+int tr_csx_transpose(tr_ctx_t ctx, const tr_csx_t *M, tr_csx_t *out);
+
+/* CSC tile -> CSR tile of the same matrix (majordim TR_COLS in, TR_ROWS out). */
+int tr_csc_to_csr(tr_ctx_t ctx, const tr_csx_t *M_csc, tr_csx_t *out);
+
+/* Sort the columns of every row in place (values follow). */
+int tr_csx_sort_rows(tr_ctx_t ctx, tr_csx_t *M);
+
+/* Device validation: *status bit0 ptr malformed, bit1 idx out of range,
+ * bit2 rows unsorted or duplicate columns. */
+int tr_csx_validate(tr_ctx_t ctx, const tr_csx_t *M, int32_t *status);
+
+/* ---- memory ---------------------------------------------------------------- */
+
+/* Release a context-owned matrix (LocalMatrix::freeBuffers,
+ * KokkosWrap.hpp:294-302); caller-owned storage is not touched.  The struct
+ * is reset to uninitialised. */
+int tr_csx_free(tr_ctx_t ctx, tr_csx_t *M);
+
+/* Build a context-owned contiguous device tile from host arrays
+ * (CSX_create_contig_device, include/utils.cuh:451-466). */
+int tr_csx_upload(tr_ctx_t ctx, int32_t majordim, int32_t nrows, int32_t ncols, int64_t nnz,
+                  const int32_t *h_ptr, const int32_t *h_idx, const double *h_val, tr_csx_t *out);
+
+/* Copy a device matrix to host arrays (ptrdim+1, nnz, nnz). */
+int tr_csx_download(tr_ctx_t ctx, const tr_csx_t *M, int32_t *h_ptr, int32_t *h_idx, double *h_val);
+
+/* Plain device memory through the context's caching pool. */
+int tr_malloc(tr_ctx_t ctx, void **ptr, size_t bytes);
+int tr_free(tr_ctx_t ctx, void *ptr);
+int tr_memcpy_h2d(tr_ctx_t ctx, void *dst, const void *src, size_t bytes);
+int tr_memcpy_d2h(tr_ctx_t ctx, void *dst, const void *src, size_t bytes);
+int tr_memcpy_d2d(tr_ctx_t ctx, void *dst, const void *src, size_t bytes);
+/* Return cached, unused pool blocks to the device. */
+int tr_pool_trim(tr_ctx_t ctx);
+/* Bytes currently handed out / cached by the pool. */
+int tr_pool_stats(tr_ctx_t ctx, size_t *in_use, size_t *cached);
+/* Test hook: make allocations beyond `bytes` in use fail with TR_ERR_OOM
+ * (0 = no limit). */
+int tr_pool_set_limit(tr_ctx_t ctx, size_t bytes);
+
+/* ---- instrumentation ------------------------------------------------------- */
+
+/* When enabled, every kernel launch on the context is bracketed by HIP
+ * events on its stream and its duration is accumulated per kernel name. */
+int tr_ctx_set_timing(tr_ctx_t ctx, int enable);
+/* Clear accumulated kernel timings. */
+int tr_ctx_reset_timing(tr_ctx_t ctx);
+/* Number of distinct kernel names timed so far. */
+int tr_ctx_timing_count(tr_ctx_t ctx);
+/* Entry i: name (copied into `name`, up to name_len bytes), launches and
+ * total milliseconds.  Synchronises the stream. */
+int tr_ctx_timing_get(tr_ctx_t ctx, int i, char *name, int name_len, int64_t *launches, double *total_ms);
+
+/* Library build string (architecture and tuning constants). */
+const char *tr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

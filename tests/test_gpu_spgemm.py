@@ -1,0 +1,184 @@
+"""GPU parity of the local multiply (tr_spgemm, tr_spgemm_panels,
+tr_symbolic, tr_spgemm_flops) against the CPU oracle.
+
+The bar: row pointers and column indices bit-exact (sorted columns) and
+values bit-exact — the HIP kernels apply every product in the oracle's
+canonical order, so max relative error is 0 (the contract allows 1e-12).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from trident_amd import gen
+from trident_amd.device import spgemm, spgemm_flops, spgemm_panels, symbolic
+from trident_amd.formats import read_bmtx
+from trident_amd._lib import TridentError, TR_ERR_INVALID, TR_ERR_OVERFLOW
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(C, ref, label=""):
+    cp, ci, cv = ref
+    assert C.nrows == len(cp) - 1, label
+    assert np.array_equal(C.ptr.astype(np.int64), cp), f"{label}: row pointers differ"
+    assert np.array_equal(C.idx, ci), f"{label}: column indices differ"
+    if not np.array_equal(C.val, cv):
+        rel = np.abs(C.val - cv) / np.maximum(np.abs(cv), 1e-300)
+        pytest.fail(f"{label}: values differ, max rel err {rel.max():.3e}")
+
+
+def run(ctx, A, B, label=""):
+    dA = ctx.upload(A)
+    dB = ctx.upload(B) if B is not A else dA
+    dC = spgemm(dA, dB)
+    C = dC.download()
+    ref = oracle.spgemm(A, B)
+    assert_same(C, ref, label)
+    dC.free()
+    dA.free()
+    if B is not A:
+        dB.free()
+    return C
+
+
+def test_er1024_plain(ctx):
+    # the CPU configuration: Erdos-Renyi n=1024, 8 columns per row, seed 0
+    A = gen.erdos_renyi(1024, 8, seed=0)
+    run(ctx, A, A, "er1024")
+
+
+@pytest.mark.parametrize("scale,ef,seed", [(8, 4, 1), (10, 8, 2), (12, 16, 1), (14, 16, 3)])
+def test_rmat(ctx, scale, ef, seed):
+    A = gen.rmat(scale, ef, seed=seed)
+    run(ctx, A, A, f"rmat s{scale}")
+
+
+def test_rmat_rectangular_pair(ctx):
+    A = gen.submatrix(gen.rmat(13, 16, seed=4), 0, 3000, 0, 8192)
+    B = gen.submatrix(gen.rmat(13, 16, seed=5), 0, 8192, 100, 5000)
+    run(ctx, A, B, "rect")
+
+
+def test_reference_fixtures(ctx):
+    for name in ("n16_full.bmtx", "n1024.bmtx"):
+        A = read_bmtx(os.path.join(GOLDEN, name))
+        run(ctx, A, A, name)
+
+
+def test_closed_forms(ctx):
+    A = gen.rmat(11, 8, seed=21)
+    n = A.nrows
+    I = gen.from_arrays(n, n, np.arange(n + 1), np.arange(n), np.ones(n))
+    C = run(ctx, I, A, "I*A")
+    assert np.array_equal(C.idx, A.idx) and np.array_equal(C.val, A.val)
+    C = run(ctx, A, I, "A*I")
+    assert np.array_equal(C.idx, A.idx) and np.array_equal(C.val, A.val)
+    Z = gen.from_arrays(n, n, np.zeros(n + 1), [], [])
+    C = run(ctx, A, Z, "A*0")
+    assert C.nnz == 0
+
+
+def test_heavy_dense_rows_multiwindow(ctx):
+    # a few rows whose products cover most columns: exercises the heavy
+    # symbolic bitmap, multi-window items and the in-step conflict ordering
+    rng = np.random.default_rng(0)
+    n = 20000
+    dense_rows = 6
+    B = gen.erdos_renyi(n, 40, seed=3)
+    rows = []
+    for r in range(dense_rows):
+        rows.append(np.sort(rng.choice(n, size=3000 + 1000 * r, replace=False)))
+    rows += [np.sort(rng.choice(n, size=3, replace=False)) for _ in range(100)]
+    ptr = np.concatenate([[0], np.cumsum([len(x) for x in rows])])
+    A = gen.from_arrays(len(rows), n, ptr, np.concatenate(rows), rng.uniform(-1, 1, ptr[-1]))
+    run(ctx, A, B, "dense rows")
+
+
+def test_heavy_multipass_wide(ctx):
+    # more than 2^20 columns: the heavy bitmap runs several column passes
+    rng = np.random.default_rng(1)
+    ncols = (1 << 20) + 5000
+    k = 300
+    bl = []
+    for r in range(k):
+        cnt = 20 if r % 7 else 4000
+        bl.append(np.sort(rng.choice(ncols, size=cnt, replace=False)))
+    bptr = np.concatenate([[0], np.cumsum([len(x) for x in bl])])
+    B = gen.from_arrays(k, ncols, bptr, np.concatenate(bl), rng.uniform(-1, 1, bptr[-1]))
+    al = [np.sort(rng.choice(k, size=int(rng.integers(1, 200)), replace=False)) for _ in range(64)]
+    aptr = np.concatenate([[0], np.cumsum([len(x) for x in al])])
+    A = gen.from_arrays(64, k, aptr, np.concatenate(al), rng.uniform(-1, 1, aptr[-1]))
+    run(ctx, A, B, "multipass")
+
+
+def test_structural_zero_and_unsorted_A(ctx):
+    # products cancelling to exactly 0 keep their entry; A's stored order
+    # (unsorted within a row) defines the accumulation order
+    A = gen.from_arrays(2, 3, [0, 2, 5], [0, 1, 2, 0, 1], [1.0, 1.0, 0.1, 0.7, -0.3])
+    B = gen.from_arrays(3, 3, [0, 2, 3, 5], [0, 2, 0, 1, 2], [2.5, 1.0, -2.5, 0.25, 3.0])
+    dA = ctx.upload(A)
+    dB = ctx.upload(B)
+    with pytest.raises(TridentError) as e:
+        spgemm(dA, dB)  # A unsorted: rejected with a clear error
+    assert e.value.code == TR_ERR_INVALID
+    dA.sort_rows()
+    A_sorted = dA.download()
+    dC = spgemm(dA, dB)
+    C = dC.download()
+    assert_same(C, oracle.spgemm(A_sorted, B), "zeros")
+    assert C.idx[:2].tolist() == [0, 2] and C.val[0] == 0.0
+
+
+def test_flops_and_symbolic(ctx):
+    A = gen.rmat(13, 16, seed=8)
+    dA = ctx.upload(A)
+    assert spgemm_flops(dA, dA) == oracle.spgemm_flops(A, A)
+    nnz, nbytes = symbolic(dA, dA)
+    assert nnz == int(oracle.spgemm_rownnz(A, A).sum())
+    assert nbytes >= nnz * 12
+
+
+def test_panels(ctx):
+    A = gen.rmat(12, 16, seed=6)
+    ref = oracle.spgemm(A, A)
+    total = int(ref[0][-1])
+    dA = ctx.upload(A)
+    cap = total // 5 + 1
+    panels, rb = spgemm_panels(dA, dA, max_panel_nnz=cap)
+    assert len(panels) >= 5 and rb[0] == 0 and rb[-1] == A.nrows
+    ptrs, idxs, vals = [], [], []
+    base = 0
+    for p, P in enumerate(panels):
+        H = P.download()
+        assert H.nrows == rb[p + 1] - rb[p] and H.nnz <= cap
+        ptrs.append(H.ptr[:-1].astype(np.int64) + base)
+        base += H.nnz
+        idxs.append(H.idx)
+        vals.append(H.val)
+        P.free()
+    ptr = np.concatenate(ptrs + [np.array([base])])
+    assert np.array_equal(ptr, ref[0])
+    assert np.array_equal(np.concatenate(idxs), ref[1])
+    assert np.array_equal(np.concatenate(vals), ref[2])
+    # a single tile cannot hold it when the cap is below nnz
+    with pytest.raises(TridentError) as e:
+        spgemm_panels(dA, dA, max_panel_nnz=cap, max_panels=2)
+    assert e.value.code == TR_ERR_OVERFLOW
+
+
+def test_invalid_operands_rejected(ctx):
+    A = gen.rmat(8, 4, seed=1)
+    B = gen.rmat(9, 4, seed=1)
+    dA = ctx.upload(A)
+    dB = ctx.upload(B)
+    with pytest.raises(TridentError) as e:
+        spgemm(dA, dB)  # 256 cols vs 512 rows
+    assert e.value.code == TR_ERR_INVALID
+    bad = gen.from_arrays(A.nrows, A.ncols, A.ptr, np.where(A.idx == A.idx.max(), 10**6, A.idx), A.val)
+    dbad = ctx.upload(bad)
+    with pytest.raises(TridentError) as e:
+        spgemm(dbad, dA)
+    assert e.value.code == TR_ERR_INVALID

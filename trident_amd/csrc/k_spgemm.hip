@@ -1,0 +1,810 @@
+// Local SpGEMM C = A*B on gfx950 (int32 CSR, fp64 values).
+//
+// Replaces LocalMatrix::spgemm / KokkosSparse::spgemm
+// (include/KokkosWrap.hpp:367-391, call at :375).  Pipeline:
+//
+//  1. row analysis   one wave per row: flops_i = sum_{k in A_i} nnz(B_k);
+//                    rows are binned by flops (empty / light classes by hash
+//                    table size / heavy).
+//  2. symbolic       light rows (flops <= kLightMaxF): one wave per row, LDS
+//                    open-addressing hash of column keys;
+//                    heavy rows: one 1024-thread block per row, a bitmap of
+//                    2^20 columns in LDS (128 KiB of the 160 KiB), popcounted
+//                    per 128-column fine block; the block packs fine blocks
+//                    into column windows of at most kWinH+127 entries and
+//                    appends them to a device work list.
+//  3. scan           row counts -> 64-bit row offsets; C is cut into int32
+//                    tiles (panels) when it exceeds INT32_MAX entries.
+//  4. numeric        light rows: one wave per row, LDS hash of (key, fp64),
+//                    A's entries walked in stored order with the lanes over
+//                    B_k (no two lanes share a column, so values need no
+//                    atomics and every output sees its products in order);
+//                    heavy rows: one wave per group of windows, products of
+//                    the window flattened over the lanes in A-order, column
+//                    conflicts inside a step applied in lane (= A) order.
+//                    Each row/window is compacted and bitonic-sorted in LDS
+//                    and streamed out.
+//
+// Every output entry therefore receives c = fma(a_ik, b_kj, c) in the order
+// of A's stored entries, starting from +0.0: the results are bit-identical to
+// the CPU oracle and do not depend on scheduling.
+#include "dev_common.h"
+#include "tr_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace tr {
+
+// tuning constants (see DESIGN notes in the repository documentation)
+constexpr int kLightMaxF = 1024;   // symbolic light/heavy cut on row flops
+constexpr int kNumLight = 6;       // light classes: T = 64 << c, c < kNumLight
+constexpr int kHeavyClass = kNumLight;
+constexpr int kNumClasses = kNumLight + 1;
+constexpr int kPassBits = 20;      // heavy bitmap covers 2^20 columns per pass
+constexpr int kFineShift = 7;      // fine block = 128 columns
+constexpr int kWinH = 384;         // window packing quantum (entries)
+constexpr int kN2T = 1024;         // heavy window hash slots (>= 2 * 511)
+constexpr int kItemWindows = 16;   // windows per heavy work item
+constexpr int kHeavyThreads = 1024;
+
+struct Window {
+    int32_t c0, c1;   // column range [c0, c1)
+    int32_t off;      // offset of the window's first entry within the row
+    int32_t cnt;      // entries in the window
+};
+
+struct Item {
+    int32_t row;
+    int32_t wbeg, wend;  // window index range
+    int32_t pad;
+};
+
+struct Panel {
+    int64_t nnz_begin;   // global offset of the panel's first entry
+    int32_t row_begin;
+    int32_t pad;
+    int32_t *idx;
+    double *val;
+};
+
+// device counters written by the symbolic phase
+struct Counters {
+    int32_t cls[kNumClasses];        // rows per class
+    int32_t pad0;
+    unsigned long long flops;        // total multiplications
+    unsigned long long heavy_flops;  // multiplications of heavy rows
+    int32_t windows;                 // windows appended
+    int32_t items;                   // items appended
+    int32_t overflow;                // capacity exceeded
+    int32_t pad1;
+};
+
+namespace {
+
+using namespace dev;
+
+__device__ __forceinline__ int32_t lower_bound_dev(const int32_t *__restrict__ a, int32_t lo, int32_t hi,
+                                                   int32_t key) {
+    while (lo < hi) {
+        int32_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// first q >= s with a[q] >= key, galloping from s (cheap when the answer is
+// close to s, as for consecutive windows)
+__device__ __forceinline__ int32_t gallop_dev(const int32_t *__restrict__ a, int32_t s, int32_t e, int32_t key) {
+    if (s >= e || a[s] >= key) return s;
+    int32_t lo = s, step = 1, hi;
+    while (true) {
+        hi = lo + step;
+        if (hi >= e) {
+            hi = e;
+            break;
+        }
+        if (a[hi] >= key) break;
+        lo = hi;
+        step <<= 1;
+    }
+    // a[lo] < key, (hi == e || a[hi] >= key)
+    while (hi - lo > 1) {
+        int32_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] < key) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+__device__ __forceinline__ int owner_lane32(int32_t excl, int32_t x) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        int probe = lo + step;
+        int32_t v = __shfl(excl, probe & 63, kWave);
+        if (probe < kWave && v <= x) lo = probe;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ const Panel &find_panel(const Panel *__restrict__ panels, int np, int32_t row) {
+    int p = 0;
+    while (p + 1 < np && panels[p + 1].row_begin <= row) ++p;
+    return panels[p];
+}
+
+// ---------------------------------------------------------------- analysis
+__global__ __launch_bounds__(256) void k_row_flops(const int32_t *__restrict__ ap, const int32_t *__restrict__ ai,
+                                                   const int32_t *__restrict__ bp, int32_t m, int32_t ncols,
+                                                   int64_t *__restrict__ flops, int32_t *__restrict__ lists,
+                                                   Counters *__restrict__ cnt) {
+    const int l = lane_id();
+    const int32_t row = blockIdx.x * 4 + threadIdx.x / kWave;
+    if (row >= m) return;
+    int64_t f = 0;
+    for (int32_t p = ap[row] + l; p < ap[row + 1]; p += kWave) {
+        int32_t k = ai[p];
+        f += bp[k + 1] - bp[k];
+    }
+    f = wave_sum(f);
+    if (l == 0) {
+        flops[row] = f;
+        if (f > 0) {
+            atomicAdd(&cnt->flops, (unsigned long long)f);
+            int c;
+            if (f <= kLightMaxF) {
+                int64_t bound = f < ncols ? f : ncols;
+                int64_t T = 64;
+                c = 0;
+                while (T < 2 * bound) {
+                    T <<= 1;
+                    ++c;
+                }
+            } else {
+                c = kHeavyClass;
+                atomicAdd(&cnt->heavy_flops, (unsigned long long)f);
+            }
+            int32_t pos = atomicAdd(&cnt->cls[c], 1);
+            lists[(int64_t)c * m + pos] = row;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- symbolic, light rows
+template <int T>
+__global__ __launch_bounds__(256) void k_sym_light(const int32_t *__restrict__ ap, const int32_t *__restrict__ ai,
+                                                   const int32_t *__restrict__ bp, const int32_t *__restrict__ bi,
+                                                   const int32_t *__restrict__ rows, int32_t nrows,
+                                                   int64_t *__restrict__ rownnz) {
+    __shared__ int32_t tabs[4][T];
+    constexpr uint32_t kLog = __builtin_ctz(T);
+    const int w = threadIdx.x / kWave, l = lane_id();
+    const int32_t li = blockIdx.x * 4 + w;
+    if (li >= nrows) return;
+    int32_t *tab = tabs[w];
+    const int32_t row = rows[li];
+    for (int t = l; t < T; t += kWave) tab[t] = kEmpty;
+    wave_sync();
+    const int32_t a0 = ap[row], a1 = ap[row + 1];
+    int cnt = 0;
+    for (int32_t pb = a0; pb < a1; pb += kWave) {
+        int32_t bs = 0, be = 0;
+        if (pb + l < a1) {
+            int32_t k = ai[pb + l];
+            bs = bp[k];
+            be = bp[k + 1];
+        }
+        const int nk = min(kWave, a1 - pb);
+        for (int t = 0; t < nk; ++t) {
+            const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
+            for (int32_t q = s + l; q < e; q += kWave) {
+                const int32_t j = bi[q];
+                uint32_t h = hash_slot(j, kLog);
+                for (int probe = 0; probe < T; ++probe) {
+                    int32_t old = atomicCAS(&tab[h], kEmpty, j);
+                    if (old == kEmpty) {
+                        ++cnt;
+                        break;
+                    }
+                    if (old == j) break;
+                    h = (h + 1) & (T - 1);
+                }
+            }
+        }
+    }
+    cnt = wave_sum(cnt);
+    if (l == 0) rownnz[row] = cnt;
+}
+
+// ---------------------------------------------------------------- symbolic, heavy rows
+__global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
+    const int32_t *__restrict__ ap, const int32_t *__restrict__ ai, const int32_t *__restrict__ bp,
+    const int32_t *__restrict__ bi, int32_t ncols, const int32_t *__restrict__ rows, int32_t nrows,
+    int64_t *__restrict__ rownnz, int emit, Window *__restrict__ wins, int32_t win_cap, Item *__restrict__ items,
+    int32_t item_cap, Counters *__restrict__ cnt) {
+    constexpr int kWords = 1 << (kPassBits - 5);   // 32768 words
+    constexpr int kFine = 1 << (kPassBits - kFineShift);  // 8192 fine blocks
+    constexpr int kPerThread = kFine / kHeavyThreads;     // 8
+    constexpr int kWavesB = kHeavyThreads / kWave;        // 16
+    __shared__ uint32_t bm[kWords];
+    __shared__ int32_t wsum[kWavesB];
+    __shared__ int32_t sh_wbase;
+
+    const int tid = threadIdx.x, w = tid / kWave, l = lane_id();
+    const int32_t row = rows[blockIdx.x];
+    const int32_t a0 = ap[row], a1 = ap[row + 1];
+    int32_t row_total = 0;
+    int32_t *fex_lds = (int32_t *)bm;              // [0, kFine) after marking
+    int32_t *first_lds = (int32_t *)bm + kFine;    // [kFine, 2 kFine)
+
+    for (int64_t base = 0; base < ncols; base += (1 << kPassBits)) {
+        const int32_t width = (int32_t)min<int64_t>(1 << kPassBits, (int64_t)ncols - base);
+        const int32_t nwords = (width + 31) >> 5;
+        const int32_t nfine = (width + (1 << kFineShift) - 1) >> kFineShift;
+        for (int t = tid; t < kWords; t += kHeavyThreads) bm[t] = 0u;
+        __syncthreads();
+        // mark
+        for (int32_t pb = a0 + w * kWave; pb < a1; pb += kWavesB * kWave) {
+            int32_t bs = 0, be = 0;
+            if (pb + l < a1) {
+                int32_t k = ai[pb + l];
+                bs = bp[k];
+                be = bp[k + 1];
+            }
+            const int nk = min(kWave, a1 - pb);
+            for (int t = 0; t < nk; ++t) {
+                const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
+                for (int32_t q = s + l; q < e; q += kWave) {
+                    int64_t jj = (int64_t)bi[q] - base;
+                    if (jj >= 0 && jj < width) atomicOr(&bm[jj >> 5], 1u << (jj & 31));
+                }
+            }
+        }
+        __syncthreads();
+        // fine-block counts, 8 consecutive fine blocks per thread
+        int32_t c[kPerThread];
+        int32_t tsum = 0;
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u) {
+            const int f = tid * kPerThread + u;
+            int32_t v = 0;
+            if (f < nfine) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int wd = f * 4 + q;
+                    if (wd < nwords) v += __popc(bm[wd]);
+                }
+            }
+            c[u] = v;
+            tsum += v;
+        }
+        // block exclusive scan of tsum
+        int32_t inc = wave_incl_scan(tsum);
+        if (l == kWave - 1) wsum[w] = inc;
+        __syncthreads();
+        int32_t off = 0, pass_total = 0;
+        for (int v = 0; v < kWavesB; ++v) {
+            if (v < w) off += wsum[v];
+            pass_total += wsum[v];
+        }
+        int32_t fex[kPerThread];
+        {
+            int32_t run = off + inc - tsum;
+#pragma unroll
+            for (int u = 0; u < kPerThread; ++u) {
+                fex[u] = run;
+                run += c[u];
+            }
+        }
+        if (pass_total == 0) {
+            __syncthreads();
+            continue;
+        }
+        if (!emit) {
+            row_total += pass_total;
+            __syncthreads();
+            continue;
+        }
+        // fine prefixes to LDS (bitmap no longer needed)
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u) fex_lds[tid * kPerThread + u] = fex[u];
+        __syncthreads();
+        // window starts: first(f) = f == 0 || fex[f]/H != fex[f-1]/H
+        int flags = 0, nflag = 0;
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u) {
+            const int f = tid * kPerThread + u;
+            bool first = false;
+            if (f < nfine) first = (f == 0) || (fex[u] / kWinH != fex_lds[f - 1] / kWinH);
+            if (first) {
+                flags |= 1 << u;
+                ++nflag;
+            }
+        }
+        int32_t finc = wave_incl_scan(nflag);
+        __syncthreads();
+        if (l == kWave - 1) wsum[w] = finc;
+        __syncthreads();
+        int32_t foff = 0, nwin = 0;
+        for (int v = 0; v < kWavesB; ++v) {
+            if (v < w) foff += wsum[v];
+            nwin += wsum[v];
+        }
+        int32_t r = foff + finc - nflag;
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u)
+            if (flags & (1 << u)) first_lds[r++] = tid * kPerThread + u;
+        if (tid == 0) {
+            int32_t nitems = (nwin + kItemWindows - 1) / kItemWindows;
+            int32_t wb = atomicAdd(&cnt->windows, nwin);
+            int32_t ib = atomicAdd(&cnt->items, nitems);
+            if (wb + nwin > win_cap || ib + nitems > item_cap) {
+                atomicOr(&cnt->overflow, 1);
+                sh_wbase = -1;
+            } else {
+                sh_wbase = wb;
+                for (int32_t it = 0; it < nitems; ++it) {
+                    Item I;
+                    I.row = row;
+                    I.wbeg = wb + it * kItemWindows;
+                    I.wend = min(wb + nwin, I.wbeg + kItemWindows);
+                    I.pad = 0;
+                    items[ib + it] = I;
+                }
+            }
+        }
+        __syncthreads();
+        const int32_t wbase = sh_wbase;
+        if (wbase >= 0) {
+            for (int32_t rr = tid; rr < nwin; rr += kHeavyThreads) {
+                const int32_t f = first_lds[rr];
+                const int32_t fn = (rr + 1 < nwin) ? first_lds[rr + 1] : nfine;
+                Window W;
+                W.c0 = (int32_t)(base + ((int64_t)f << kFineShift));
+                W.c1 = (int32_t)min<int64_t>(base + ((int64_t)fn << kFineShift), base + width);
+                W.off = row_total + fex_lds[f];
+                W.cnt = (fn < nfine ? fex_lds[fn] : pass_total) - fex_lds[f];
+                wins[wbase + rr] = W;
+            }
+        }
+        row_total += pass_total;
+        __syncthreads();
+    }
+    if (tid == 0) rownnz[row] = row_total;
+}
+
+// per item: number of cursor slots it needs (multi-window items only)
+__global__ void k_item_cursor_len(const Item *__restrict__ items, const Counters *__restrict__ cnt,
+                                  const int32_t *__restrict__ ap, int32_t cap, int32_t *__restrict__ len) {
+    int32_t it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= cap) return;
+    int32_t v = 0;
+    if (it < cnt->items) {
+        Item I = items[it];
+        if (I.wend - I.wbeg > 1) v = ap[I.row + 1] - ap[I.row];
+    }
+    len[it] = v;
+}
+
+// ---------------------------------------------------------------- numeric, light rows
+template <int T>
+__global__ __launch_bounds__(256) void k_num_light(const int32_t *__restrict__ ap, const int32_t *__restrict__ ai,
+                                                   const double *__restrict__ av, const int32_t *__restrict__ bp,
+                                                   const int32_t *__restrict__ bi, const double *__restrict__ bv,
+                                                   const int32_t *__restrict__ rows, int32_t nrows,
+                                                   const int64_t *__restrict__ rowptr, const Panel *__restrict__ panels,
+                                                   int npanels, Counters *__restrict__ cnt) {
+    __shared__ int32_t keys_s[4][T];
+    __shared__ double vals_s[4][T];
+    constexpr uint32_t kLog = __builtin_ctz(T);
+    const int w = threadIdx.x / kWave, l = lane_id();
+    const int32_t li = blockIdx.x * 4 + w;
+    if (li >= nrows) return;
+    int32_t *keys = keys_s[w];
+    double *vals = vals_s[w];
+    const int32_t row = rows[li];
+    for (int t = l; t < T; t += kWave) keys[t] = kEmpty;
+    wave_sync();
+    const int32_t a0 = ap[row], a1 = ap[row + 1];
+    for (int32_t pb = a0; pb < a1; pb += kWave) {
+        int32_t bs = 0, be = 0;
+        double a = 0.0;
+        if (pb + l < a1) {
+            int32_t k = ai[pb + l];
+            a = av[pb + l];
+            bs = bp[k];
+            be = bp[k + 1];
+        }
+        const int nk = min(kWave, a1 - pb);
+        for (int t = 0; t < nk; ++t) {
+            const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
+            const double at = __shfl(a, t, kWave);
+            for (int32_t q = s + l; q < e; q += kWave) {
+                const int32_t j = bi[q];
+                const double b = bv[q];
+                uint32_t h = hash_slot(j, kLog);
+                for (int probe = 0; probe < T; ++probe) {
+                    int32_t old = atomicCAS(&keys[h], kEmpty, j);
+                    if (old == kEmpty) {
+                        vals[h] = __fma_rn(at, b, 0.0);
+                        break;
+                    }
+                    if (old == j) {
+                        vals[h] = __fma_rn(at, b, vals[h]);
+                        break;
+                    }
+                    h = (h + 1) & (T - 1);
+                }
+            }
+        }
+    }
+    wave_sync();
+    const int n = wave_compact_lds<double>(keys, vals, T);
+    wave_sort_lds<double>(keys, vals, n);
+    const int64_t rs = rowptr[row];
+    const int expect = (int)(rowptr[row + 1] - rs);
+    if (l == 0 && n != expect) atomicOr(&cnt->overflow, 2);
+    const int nw = n < expect ? n : expect;  // never write past the row
+    const Panel &P = find_panel(panels, npanels, row);
+    int32_t *oi = P.idx + (rs - P.nnz_begin);
+    double *ov = P.val + (rs - P.nnz_begin);
+    for (int t = l; t < nw; t += kWave) {
+        oi[t] = keys[t];
+        ov[t] = vals[t];
+    }
+}
+
+// ---------------------------------------------------------------- numeric, heavy rows (windows)
+__global__ __launch_bounds__(kWave) void k_num_heavy(
+    const int32_t *__restrict__ ap, const int32_t *__restrict__ ai, const double *__restrict__ av,
+    const int32_t *__restrict__ bp, const int32_t *__restrict__ bi, const double *__restrict__ bv, int32_t ncols,
+    const Item *__restrict__ items, const Window *__restrict__ wins, const int64_t *__restrict__ cursor_base,
+    int32_t *__restrict__ cursors, const int64_t *__restrict__ rowptr, const Panel *__restrict__ panels,
+    int npanels, Counters *__restrict__ cnt) {
+    constexpr int T = kN2T;
+    constexpr uint32_t kLog = __builtin_ctz(T);
+    __shared__ int32_t keys[T];
+    __shared__ double vals[T];
+    __shared__ uint32_t tags[T];
+    const int l = lane_id();
+    const Item I = items[blockIdx.x];
+    const int32_t row = I.row;
+    const int32_t a0 = ap[row], a1 = ap[row + 1];
+    int32_t *cur = cursors + cursor_base[blockIdx.x];
+    const int64_t rs = rowptr[row];
+    const Panel &P = find_panel(panels, npanels, row);
+    int32_t *oi_row = P.idx + (rs - P.nnz_begin);
+    double *ov_row = P.val + (rs - P.nnz_begin);
+
+    for (int t = l; t < T; t += kWave) {
+        keys[t] = kEmpty;
+        tags[t] = 0xffffffffu;
+    }
+    wave_sync();
+    for (int32_t wi = I.wbeg; wi < I.wend; ++wi) {
+        const Window W = wins[wi];
+        const bool full = (W.c0 == 0 && W.c1 >= ncols);
+        const bool first = (wi == I.wbeg), last = (wi == I.wend - 1);
+        for (int32_t pb = a0; pb < a1; pb += kWave) {
+            const int32_t p = pb + l;
+            int32_t s = 0, e = 0;
+            double a = 0.0;
+            if (p < a1) {
+                const int32_t k = ai[p];
+                a = av[p];
+                const int32_t bs = bp[k], be = bp[k + 1];
+                if (full) {
+                    s = bs;
+                    e = be;
+                } else {
+                    s = first ? lower_bound_dev(bi, bs, be, W.c0) : cur[p - a0];
+                    e = gallop_dev(bi, s, be, W.c1);
+                    if (!last) cur[p - a0] = e;
+                }
+            }
+            const int32_t len = e - s;
+            const int32_t incl = wave_incl_scan(len);
+            const int32_t excl = incl - len;
+            const int32_t total = __shfl(incl, kWave - 1, kWave);
+            for (int32_t x0 = 0; x0 < total; x0 += kWave) {
+                const int32_t x = x0 + l;
+                bool act = x < total;
+                const int o = owner_lane32(excl, x);
+                const int32_t so = __shfl(s, o, kWave);
+                const int32_t eo = __shfl(excl, o, kWave);
+                const double ao = __shfl(a, o, kWave);
+                int32_t j = 0;
+                double b = 0.0;
+                uint32_t h = 0;
+                if (act) {
+                    const int32_t q = so + (x - eo);
+                    j = bi[q];
+                    b = bv[q];
+                    h = hash_slot(j, kLog);
+                    bool placed = false;
+                    for (int probe = 0; probe < T; ++probe) {
+                        int32_t old = atomicCAS(&keys[h], kEmpty, j);
+                        if (old == kEmpty) {
+                            vals[h] = 0.0;
+                            placed = true;
+                            break;
+                        }
+                        if (old == j) {
+                            placed = true;
+                            break;
+                        }
+                        h = (h + 1) & (T - 1);
+                    }
+                    if (!placed) {
+                        // table full: only reachable if symbolic counts are wrong
+                        atomicOr(&cnt->overflow, 8);
+                        act = false;
+                    }
+                }
+                wave_sync();
+                // apply in lane order among lanes that share a slot
+                bool pending = act;
+                while (ballot(pending)) {
+                    if (pending) atomicMin(&tags[h], (uint32_t)l);
+                    wave_sync();
+                    const bool mine = pending && tags[h] == (uint32_t)l;
+                    if (mine) vals[h] = __fma_rn(ao, b, vals[h]);
+                    wave_sync();
+                    if (mine) {
+                        tags[h] = 0xffffffffu;
+                        pending = false;
+                    }
+                    wave_sync();
+                }
+            }
+        }
+        wave_sync();
+        const int n = wave_compact_lds<double>(keys, vals, T);
+        wave_sort_lds<double>(keys, vals, n);
+        if (l == 0 && n != W.cnt) atomicOr(&cnt->overflow, 4);
+        const int nw = n < W.cnt ? n : W.cnt;  // never write past the window
+        int32_t *oi = oi_row + W.off;
+        double *ov = ov_row + W.off;
+        for (int t = l; t < nw; t += kWave) {
+            oi[t] = keys[t];
+            ov[t] = vals[t];
+        }
+        wave_sync();
+        for (int t = l; t < T; t += kWave) keys[t] = kEmpty;
+        wave_sync();
+    }
+}
+
+template <int T>
+void launch_sym_light(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, const int32_t *rows, int32_t n,
+                      int64_t *rownnz) {
+    TR_LAUNCH(ctx, "sym_light", k_sym_light<T>, div_up((uint64_t)n, 4), 256, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
+              B->idx_vec, rows, n, rownnz);
+}
+
+template <int T>
+void launch_num_light(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, const int32_t *rows, int32_t n,
+                      const int64_t *rowptr, const Panel *panels, int np, Counters *cnt) {
+    TR_LAUNCH(ctx, "num_light", k_num_light<T>, div_up((uint64_t)n, 4), 256, 0, A->ptr_vec, A->idx_vec, A->val,
+              B->ptr_vec, B->idx_vec, B->val, rows, n, rowptr, panels, np, cnt);
+}
+
+struct Plan {
+    int32_t m = 0, n = 0;
+    int64_t *flops = nullptr;
+    int32_t *lists = nullptr;
+    Counters *cnt = nullptr;
+    Counters h{};
+    int64_t *rownnz = nullptr;
+    int64_t *rowptr = nullptr;
+    Window *wins = nullptr;
+    Item *items = nullptr;
+    int32_t win_cap = 0, item_cap = 0;
+    int64_t nnz = 0;
+};
+
+void validate_operands(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B) {
+    check_csx(A, "A");
+    check_csx(B, "B");
+    TR_REQUIRE(A->majordim == TR_ROWS && B->majordim == TR_ROWS, "spgemm operands must be CSR (TR_ROWS)");
+    TR_REQUIRE(A->ncols == B->nrows, "spgemm: A.ncols != B.nrows");
+    int32_t *st = (int32_t *)scratch(ctx, S_PANELS, 64);
+    TR_HIP(hipMemsetAsync(st, 0, 2 * sizeof(int32_t), ctx->stream));
+    launch_validate(ctx, A, st);
+    launch_validate(ctx, B, st + 1);
+    int32_t hs[2];
+    read_array(ctx, hs, st, sizeof(hs));
+    if (hs[0] & 3) fail(TR_ERR_INVALID, "spgemm: A is not a valid CSR (ptr or column index out of range)");
+    if (hs[1] & 3) fail(TR_ERR_INVALID, "spgemm: B is not a valid CSR (ptr or column index out of range)");
+    if (hs[1] & 4) fail(TR_ERR_INVALID, "spgemm: B has unsorted or duplicate columns; call tr_csx_sort_rows");
+    if (hs[0] & 4) fail(TR_ERR_INVALID, "spgemm: A has unsorted or duplicate columns; call tr_csx_sort_rows");
+}
+
+// row analysis + symbolic; leaves per-row nnz and 64-bit offsets in the plan
+void symbolic(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, Plan &pl, bool emit) {
+    pl.m = A->nrows;
+    pl.n = B->ncols;
+    const int32_t m = pl.m;
+    pl.flops = (int64_t *)scratch(ctx, S_FLOPS, sizeof(int64_t) * ((size_t)m + 1));
+    pl.lists = (int32_t *)scratch(ctx, S_BINS, sizeof(int32_t) * (size_t)kNumClasses * ((size_t)m + 1));
+    pl.cnt = (Counters *)scratch(ctx, S_BINCNT, sizeof(Counters));
+    pl.rownnz = (int64_t *)scratch(ctx, S_ROWNNZ, sizeof(int64_t) * ((size_t)m + 1));
+    pl.rowptr = (int64_t *)scratch(ctx, S_ROWPTR64, sizeof(int64_t) * ((size_t)m + 1));
+    TR_HIP(hipMemsetAsync(pl.cnt, 0, sizeof(Counters), ctx->stream));
+    TR_HIP(hipMemsetAsync(pl.rownnz, 0, sizeof(int64_t) * ((size_t)m + 1), ctx->stream));
+    if (m > 0 && A->nnz > 0 && B->nnz > 0) {
+        TR_LAUNCH(ctx, "row_flops", k_row_flops, div_up((uint64_t)m, 4), 256, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
+                  m, pl.n, pl.flops, pl.lists, pl.cnt);
+    }
+    read_array(ctx, &pl.h, pl.cnt, sizeof(Counters));
+    const int32_t *L = pl.lists;
+    const int64_t M = m;
+    if (pl.h.cls[0]) launch_sym_light<64>(ctx, A, B, L + 0 * M, pl.h.cls[0], pl.rownnz);
+    if (pl.h.cls[1]) launch_sym_light<128>(ctx, A, B, L + 1 * M, pl.h.cls[1], pl.rownnz);
+    if (pl.h.cls[2]) launch_sym_light<256>(ctx, A, B, L + 2 * M, pl.h.cls[2], pl.rownnz);
+    if (pl.h.cls[3]) launch_sym_light<512>(ctx, A, B, L + 3 * M, pl.h.cls[3], pl.rownnz);
+    if (pl.h.cls[4]) launch_sym_light<1024>(ctx, A, B, L + 4 * M, pl.h.cls[4], pl.rownnz);
+    if (pl.h.cls[5]) launch_sym_light<2048>(ctx, A, B, L + 5 * M, pl.h.cls[5], pl.rownnz);
+    const int32_t nh = pl.h.cls[kHeavyClass];
+    if (nh) {
+        if (emit) {
+            // windows per pass <= ceil(pass_nnz / kWinH) <= pass_flops / kWinH + 1
+            const int64_t passes = ((int64_t)pl.n + (1 << kPassBits) - 1) >> kPassBits;
+            int64_t wcap = (int64_t)(pl.h.heavy_flops / kWinH) + (int64_t)nh * passes + 16;
+            int64_t icap = wcap / kItemWindows + (int64_t)nh * passes + 16;
+            TR_REQUIRE(wcap < INT32_MAX && icap < INT32_MAX, "spgemm: heavy work list exceeds int32 capacity");
+            pl.win_cap = (int32_t)wcap;
+            pl.item_cap = (int32_t)icap;
+            pl.wins = (Window *)scratch(ctx, S_WINDOWS, sizeof(Window) * (size_t)wcap);
+            pl.items = (Item *)scratch(ctx, S_ITEMS, sizeof(Item) * (size_t)icap);
+        }
+        TR_LAUNCH(ctx, "sym_heavy", k_sym_heavy, (uint32_t)nh, kHeavyThreads, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
+                  B->idx_vec, pl.n, L + (int64_t)kHeavyClass * M, nh, pl.rownnz, emit ? 1 : 0, pl.wins, pl.win_cap,
+                  pl.items, pl.item_cap, pl.cnt);
+    }
+    launch_exclusive_scan_i64(ctx, pl.rownnz, pl.rowptr, m);
+}
+
+}  // namespace
+
+int64_t spgemm_flops(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B) {
+    validate_operands(ctx, A, B);
+    Plan pl;
+    const int32_t m = A->nrows;
+    pl.flops = (int64_t *)scratch(ctx, S_FLOPS, sizeof(int64_t) * ((size_t)m + 1));
+    pl.lists = (int32_t *)scratch(ctx, S_BINS, sizeof(int32_t) * (size_t)kNumClasses * ((size_t)m + 1));
+    pl.cnt = (Counters *)scratch(ctx, S_BINCNT, sizeof(Counters));
+    TR_HIP(hipMemsetAsync(pl.cnt, 0, sizeof(Counters), ctx->stream));
+    if (m > 0 && A->nnz > 0 && B->nnz > 0)
+        TR_LAUNCH(ctx, "row_flops", k_row_flops, div_up((uint64_t)m, 4), 256, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
+                  m, B->ncols, pl.flops, pl.lists, pl.cnt);
+    read_array(ctx, &pl.h, pl.cnt, sizeof(Counters));
+    return (int64_t)pl.h.flops;
+}
+
+void spgemm_symbolic_only(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *nnz, size_t *bytes) {
+    validate_operands(ctx, A, B);
+    Plan pl;
+    symbolic(ctx, A, B, pl, false);
+    int64_t total = read_scalar<int64_t>(ctx, pl.rowptr + pl.m);
+    if (nnz) *nnz = total;
+    if (bytes) {
+        // output tile + work lists + cursors bound + row arrays
+        size_t out = (size_t)total * 12 + ((size_t)pl.m + 1) * 4;
+        size_t heavy = (size_t)(pl.h.heavy_flops / kWinH + 16) * (sizeof(Window) + sizeof(Item) / kItemWindows);
+        size_t rows = ((size_t)pl.m + 1) * (8 * 3 + 4 * kNumClasses);
+        *bytes = out + heavy + rows + (size_t)A->nnz * 4;
+    }
+}
+
+int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max_panel_nnz, tr_csx_t *panels,
+                  int32_t *row_begin, int32_t max_panels, int32_t *npanels) {
+    TR_REQUIRE(panels && row_begin && npanels && max_panels >= 1, "spgemm: null panel outputs");
+    TR_REQUIRE(max_panel_nnz >= 1 && max_panel_nnz <= INT32_MAX, "spgemm: max_panel_nnz must be in [1, INT32_MAX]");
+    validate_operands(ctx, A, B);
+    Plan pl;
+    symbolic(ctx, A, B, pl, true);
+    const int32_t m = pl.m;
+    const int32_t nh = pl.h.cls[kHeavyClass];
+
+    // cursor slots of multi-window items
+    int64_t *cbase = nullptr;
+    int32_t *clen = nullptr;
+    if (nh) {
+        clen = (int32_t *)scratch(ctx, S_MISC, sizeof(int32_t) * ((size_t)pl.item_cap + 1));
+        cbase = (int64_t *)scratch(ctx, S_MISC2, sizeof(int64_t) * ((size_t)pl.item_cap + 1));
+        TR_LAUNCH(ctx, "item_cursor_len", k_item_cursor_len, div_up((uint64_t)pl.item_cap, 256), 256, 0, pl.items,
+                  pl.cnt, A->ptr_vec, pl.item_cap, clen);
+        launch_exclusive_scan_i32_to_i64(ctx, clen, cbase, pl.item_cap);
+    }
+    // one read-back: totals
+    struct Tot {
+        int64_t nnz;
+        int64_t cursors;
+    } *ht = (Tot *)pinned(ctx, sizeof(Tot) + sizeof(Counters));
+    TR_HIP(hipMemcpyAsync(&ht->nnz, pl.rowptr + m, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (nh)
+        TR_HIP(hipMemcpyAsync(&ht->cursors, cbase + pl.item_cap, sizeof(int64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    Counters *hc = (Counters *)(ht + 1);
+    TR_HIP(hipMemcpyAsync(hc, pl.cnt, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+    TR_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t total = ht->nnz;
+    const int64_t ncur = nh ? ht->cursors : 0;
+    pl.h = *hc;
+    if (pl.h.overflow & 1) fail(TR_ERR_HIP, "spgemm: heavy work list capacity exceeded (internal bound violated)");
+
+    // panel split
+    std::vector<int32_t> rb;
+    rb.push_back(0);
+    if (total > max_panel_nnz) {
+        std::vector<int64_t> hrp((size_t)m + 1);
+        read_array(ctx, hrp.data(), pl.rowptr, sizeof(int64_t) * ((size_t)m + 1));
+        int64_t start = 0;
+        for (int32_t r = 0; r < m; ++r) {
+            if (hrp[r + 1] - hrp[r] > max_panel_nnz)
+                fail(TR_ERR_OVERFLOW, "spgemm: a single row of C exceeds the panel capacity");
+            if (hrp[r + 1] - start > max_panel_nnz) {
+                rb.push_back(r);
+                start = hrp[r];
+            }
+        }
+    }
+    rb.push_back(m);
+    const int32_t np = (int32_t)rb.size() - 1;
+    if (np > max_panels) {
+        *npanels = np;
+        fail(TR_ERR_OVERFLOW, "spgemm: nnz(C) = " + std::to_string(total) + " needs " + std::to_string(np) +
+                                  " panels of at most " + std::to_string(max_panel_nnz) + " entries");
+    }
+    std::vector<int64_t> nb(np + 1);
+    {
+        if (np == 1) {
+            nb[0] = 0;
+            nb[1] = total;
+        } else {
+            std::vector<int64_t> hrp((size_t)m + 1);
+            read_array(ctx, hrp.data(), pl.rowptr, sizeof(int64_t) * ((size_t)m + 1));
+            for (int32_t p = 0; p <= np; ++p) nb[p] = hrp[rb[p]];
+        }
+    }
+    std::vector<Panel> hp(np);
+    for (int32_t p = 0; p < np; ++p) {
+        tr_csx_t *C = &panels[p];
+        alloc_tile(ctx, TR_ROWS, rb[p + 1] - rb[p], pl.n, nb[p + 1] - nb[p], C);
+        launch_rowptr_i64_to_i32(ctx, pl.rowptr, rb[p], rb[p + 1], C->ptr_vec);
+        hp[p].nnz_begin = nb[p];
+        hp[p].row_begin = rb[p];
+        hp[p].pad = 0;
+        hp[p].idx = C->idx_vec;
+        hp[p].val = C->val;
+        row_begin[p] = rb[p];
+    }
+    row_begin[np] = m;
+    *npanels = np;
+    Panel *dp = (Panel *)scratch(ctx, S_PANELS, sizeof(Panel) * (size_t)np);
+    TR_HIP(hipMemcpyAsync(dp, hp.data(), sizeof(Panel) * (size_t)np, hipMemcpyHostToDevice, ctx->stream));
+
+    // numeric
+    const int32_t *L = pl.lists;
+    const int64_t M = m;
+    if (pl.h.cls[0]) launch_num_light<64>(ctx, A, B, L + 0 * M, pl.h.cls[0], pl.rowptr, dp, np, pl.cnt);
+    if (pl.h.cls[1]) launch_num_light<128>(ctx, A, B, L + 1 * M, pl.h.cls[1], pl.rowptr, dp, np, pl.cnt);
+    if (pl.h.cls[2]) launch_num_light<256>(ctx, A, B, L + 2 * M, pl.h.cls[2], pl.rowptr, dp, np, pl.cnt);
+    if (pl.h.cls[3]) launch_num_light<512>(ctx, A, B, L + 3 * M, pl.h.cls[3], pl.rowptr, dp, np, pl.cnt);
+    if (pl.h.cls[4]) launch_num_light<1024>(ctx, A, B, L + 4 * M, pl.h.cls[4], pl.rowptr, dp, np, pl.cnt);
+    if (pl.h.cls[5]) launch_num_light<2048>(ctx, A, B, L + 5 * M, pl.h.cls[5], pl.rowptr, dp, np, pl.cnt);
+    if (nh && pl.h.items > 0) {
+        int32_t *cursors = (int32_t *)scratch(ctx, S_CURSORS, sizeof(int32_t) * (size_t)std::max<int64_t>(ncur, 1));
+        TR_LAUNCH(ctx, "num_heavy", k_num_heavy, (uint32_t)pl.h.items, kWave, 0, A->ptr_vec, A->idx_vec, A->val,
+                  B->ptr_vec, B->idx_vec, B->val, pl.n, pl.items, pl.wins, cbase, cursors, pl.rowptr, dp, np, pl.cnt);
+    }
+    // internal consistency flags (symbolic vs numeric counts)
+    int32_t ov = read_scalar<int32_t>(ctx, &pl.cnt->overflow);
+    if (ov & 6) fail(TR_ERR_HIP, "spgemm: numeric row counts disagree with the symbolic phase (internal error)");
+    return TR_OK;
+}
+
+}  // namespace tr

@@ -1,0 +1,191 @@
+// Internal host-side definitions of libtrident_hip: context, device memory
+// pool, error propagation and launch instrumentation.  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/trident_hip.h"
+
+namespace tr {
+
+// Internal errors are thrown and converted to status codes at the C-ABI
+// boundary (api.hip), so the library never exits the process.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg);
+
+#define TR_HIP(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess)                                                                     \
+            ::tr::fail(_e == hipErrorOutOfMemory ? TR_ERR_OOM : TR_ERR_HIP,                       \
+                       std::string(#expr) + ": " + hipGetErrorString(_e));                         \
+    } while (0)
+
+#define TR_REQUIRE(cond, msg)                                                                     \
+    do {                                                                                          \
+        if (!(cond)) ::tr::fail(TR_ERR_INVALID, msg);                                             \
+    } while (0)
+
+// Caching device allocator.  All work of a context is ordered on one stream,
+// so a block released by stream-ordered work can be handed to later work on
+// the same stream without synchronising.
+class Pool {
+  public:
+    Pool() = default;
+    ~Pool();
+    void *alloc(size_t bytes);
+    void release(void *p);
+    bool owns(void *p) const { return live_.count(p) != 0; }
+    size_t size_of(void *p) const;
+    void trim();
+    size_t in_use() const { return in_use_; }
+    size_t cached() const { return cached_; }
+    void set_limit(size_t bytes) { limit_ = bytes; }
+
+  private:
+    std::multimap<size_t, void *> free_;
+    std::unordered_map<void *, size_t> live_;
+    size_t in_use_ = 0, cached_ = 0, limit_ = 0;
+};
+
+struct TimingRecord {
+    const char *name;
+    hipEvent_t start, stop;
+};
+
+struct KernelStat {
+    int64_t launches = 0;
+    double total_ms = 0.0;
+};
+
+}  // namespace tr
+
+struct tr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string last_error;
+    tr::Pool pool;
+    int cu_count = 256;
+
+    // Grow-only scratch buffers keyed by slot id (stream ordered reuse).
+    std::vector<void *> scratch_ptr;
+    std::vector<size_t> scratch_size;
+
+    // Pinned host staging for small device->host reads.
+    void *pinned = nullptr;
+    size_t pinned_size = 0;
+
+    // Launch instrumentation.
+    bool timing = false;
+    std::vector<tr::TimingRecord> pending;
+    std::vector<hipEvent_t> free_events;
+    std::map<std::string, tr::KernelStat> stats;
+
+    explicit tr_ctx(int dev) : device(dev) {}
+};
+
+namespace tr {
+
+// Scratch slots used by the kernels of one call.
+enum ScratchSlot : int {
+    S_FLOPS = 0,
+    S_ROWNNZ,
+    S_ROWPTR64,
+    S_BINS,
+    S_BINCNT,
+    S_SCAN_TMP,
+    S_WINDOWS,
+    S_ITEMS,
+    S_CURSORS,
+    S_MISC,
+    S_MISC2,
+    S_PANELS,
+    S_TMP_CSX,
+    S_NUM_SLOTS
+};
+
+void *scratch(tr_ctx *ctx, int slot, size_t bytes);
+void *pinned(tr_ctx *ctx, size_t bytes);
+
+// Launch bracketing for per-kernel timing.
+struct TimerScope {
+    tr_ctx *ctx;
+    const char *name;
+    hipEvent_t ev0 = nullptr;
+    TimerScope(tr_ctx *c, const char *n);
+    ~TimerScope();
+};
+void fold_timing(tr_ctx *ctx);
+
+#define TR_LAUNCH(ctx, name, kernel, grid, block, shmem, ...)                                     \
+    do {                                                                                          \
+        dim3 _g(grid), _b(block);                                                                 \
+        if (_g.x > 0 && _g.y > 0 && _g.z > 0) {                                                   \
+            ::tr::TimerScope _ts((ctx), (name));                                                  \
+            hipLaunchKernelGGL(kernel, _g, _b, (shmem), (ctx)->stream, __VA_ARGS__);               \
+            TR_HIP(hipGetLastError());                                                            \
+        }                                                                                         \
+    } while (0)
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+inline int32_t ptrdim(const tr_csx_t *m) { return m->majordim == TR_ROWS ? m->nrows : m->ncols; }
+inline int32_t otherdim(const tr_csx_t *m) { return m->majordim == TR_ROWS ? m->ncols : m->nrows; }
+
+// Allocate a context-owned contiguous tile [val | idx | ptr] and fill the
+// descriptor (ptr array not initialised).
+void alloc_tile(tr_ctx *ctx, int32_t majordim, int32_t nrows, int32_t ncols, int64_t nnz, tr_csx_t *out);
+void release_csx(tr_ctx *ctx, tr_csx_t *m);
+void check_csx(const tr_csx_t *m, const char *what);
+
+// ---- launchers implemented in the kernel translation units ----
+
+// k_util.hip
+void launch_exclusive_scan_i64(tr_ctx *ctx, const int64_t *in, int64_t *out, int64_t n);  // out has n+1
+void launch_exclusive_scan_i32_to_i64(tr_ctx *ctx, const int32_t *in, int64_t *out, int64_t n);
+void launch_validate(tr_ctx *ctx, const tr_csx_t *m, int32_t *d_status);
+void launch_fill_i32(tr_ctx *ctx, int32_t *p, int32_t v, int64_t n);
+void launch_transpose(tr_ctx *ctx, const tr_csx_t *m, tr_csx_t *out);
+void launch_sort_rows(tr_ctx *ctx, tr_csx_t *m);
+void launch_rowptr_i64_to_i32(tr_ctx *ctx, const int64_t *rp64, int32_t r0, int32_t r1, int32_t *out);
+
+// k_bitmask.hip
+void launch_bitmask_gen(tr_ctx *ctx, const int32_t *ptr, int32_t n, uint64_t *mask);
+void launch_bitmask_and(tr_ctx *ctx, const uint64_t *a, const uint64_t *b, int32_t words, uint64_t *out);
+int64_t compress(tr_ctx *ctx, const tr_csx_t *m, const uint64_t *mask, tr_csx_t *out);
+
+// k_spgemm.hip
+struct SpgemmPlan;
+int64_t spgemm_flops(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B);
+// Runs the symbolic phase and returns nnz(C) (and bytes needed).
+void spgemm_symbolic_only(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *nnz, size_t *bytes);
+// Full product into panels.
+int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max_panel_nnz,
+                  tr_csx_t *panels, int32_t *row_begin, int32_t max_panels, int32_t *npanels);
+
+// k_spadd.hip
+void spadd(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, tr_csx_t *out);
+
+// Read a device value through pinned staging (synchronises the stream).
+template <typename T>
+T read_scalar(tr_ctx *ctx, const T *dptr) {
+    T *h = (T *)pinned(ctx, sizeof(T));
+    TR_HIP(hipMemcpyAsync(h, dptr, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    TR_HIP(hipStreamSynchronize(ctx->stream));
+    return *h;
+}
+void read_array(tr_ctx *ctx, void *host, const void *dptr, size_t bytes);
+
+}  // namespace tr
