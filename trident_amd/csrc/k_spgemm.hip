@@ -43,8 +43,11 @@ constexpr int kHeavyClass = kNumLight;
 constexpr int kNumClasses = kNumLight + 1;
 constexpr int kPassBits = 20;      // heavy bitmap covers 2^20 columns per pass
 constexpr int kFineShift = 7;      // fine block = 128 columns
-constexpr int kWinH = 384;         // window packing quantum (entries)
-constexpr int kN2T = 1024;         // heavy window hash slots (>= 2 * 511)
+constexpr int kWinH = 384;         // default window packing quantum (entries)
+constexpr int kN2T = 1024;         // default heavy window hash slots (>= 2 * (kWinH + 127))
+constexpr int kWinMaxWidth = 32768;                       // window width cap (columns)
+constexpr int kWidthShift = 15 - kFineShift;               // fine blocks per width quantum: 2^8
+constexpr int kBmWords = kWinMaxWidth / 64;                // presence words per window
 constexpr int kItemWindows = 16;   // windows per heavy work item
 constexpr int kHeavyThreads = 1024;
 
@@ -129,6 +132,36 @@ __device__ __forceinline__ int owner_lane32(int32_t excl, int32_t x) {
     return lo;
 }
 
+// Visit the column indices of B row segment [s, e) with the wave's lanes,
+// four loads in flight per lane before the first is consumed.
+template <typename F>
+__device__ __forceinline__ void stream_cols(const int32_t *__restrict__ bi, int32_t s, int32_t e, F &&f) {
+    int32_t q = s + lane_id();
+    for (; q + 3 * kWave < e; q += 4 * kWave) {
+        const int32_t j0 = bi[q], j1 = bi[q + kWave], j2 = bi[q + 2 * kWave], j3 = bi[q + 3 * kWave];
+        f(j0);
+        f(j1);
+        f(j2);
+        f(j3);
+    }
+    for (; q < e; q += kWave) f(bi[q]);
+}
+
+// Same with the values (all entries of one B row have distinct columns, so
+// the order in which one lane applies them does not matter).
+template <typename F>
+__device__ __forceinline__ void stream_entries(const int32_t *__restrict__ bi, const double *__restrict__ bv,
+                                               int32_t s, int32_t e, F &&f) {
+    int32_t q = s + lane_id();
+    for (; q + kWave < e; q += 2 * kWave) {
+        const int32_t j0 = bi[q], j1 = bi[q + kWave];
+        const double b0 = bv[q], b1 = bv[q + kWave];
+        f(j0, b0);
+        f(j1, b1);
+    }
+    for (; q < e; q += kWave) f(bi[q], bv[q]);
+}
+
 __device__ __forceinline__ const Panel &find_panel(const Panel *__restrict__ panels, int np, int32_t row) {
     int p = 0;
     while (p + 1 < np && panels[p + 1].row_begin <= row) ++p;
@@ -136,40 +169,76 @@ __device__ __forceinline__ const Panel &find_panel(const Panel *__restrict__ pan
 }
 
 // ---------------------------------------------------------------- analysis
-__global__ __launch_bounds__(256) void k_row_flops(const int32_t *__restrict__ ap, const int32_t *__restrict__ ai,
-                                                   const int32_t *__restrict__ bp, int32_t m, int32_t ncols,
-                                                   int64_t *__restrict__ flops, int32_t *__restrict__ lists,
-                                                   Counters *__restrict__ cnt) {
-    const int l = lane_id();
-    const int32_t row = blockIdx.x * 4 + threadIdx.x / kWave;
-    if (row >= m) return;
-    int64_t f = 0;
-    for (int32_t p = ap[row] + l; p < ap[row + 1]; p += kWave) {
-        int32_t k = ai[p];
-        f += bp[k + 1] - bp[k];
+// 256 rows per block: a thread per row for short rows, the whole wave for
+// rows with more than 32 entries; class counters are aggregated in LDS so
+// there is one global atomic per class per block.
+constexpr int kRowsPerBlock = 256;
+
+__global__ __launch_bounds__(kRowsPerBlock) void k_row_flops(
+    const int32_t *__restrict__ ap, const int32_t *__restrict__ ai, const int32_t *__restrict__ bp, int32_t m,
+    int32_t ncols, int64_t *__restrict__ flops, int32_t *__restrict__ lists, Counters *__restrict__ cnt) {
+    __shared__ int32_t s_cnt[kNumClasses], s_base[kNumClasses];
+    __shared__ unsigned long long s_f, s_hf;
+    const int tid = threadIdx.x, l = lane_id();
+    const int32_t row = blockIdx.x * kRowsPerBlock + tid;
+    if (tid < kNumClasses) s_cnt[tid] = 0;
+    if (tid == 0) s_f = s_hf = 0ull;
+    __syncthreads();
+    int32_t a0 = 0, a1 = 0;
+    if (row < m) {
+        a0 = ap[row];
+        a1 = ap[row + 1];
     }
-    f = wave_sum(f);
-    if (l == 0) {
-        flops[row] = f;
-        if (f > 0) {
-            atomicAdd(&cnt->flops, (unsigned long long)f);
-            int c;
-            if (f <= kLightMaxF) {
-                int64_t bound = f < ncols ? f : ncols;
-                int64_t T = 64;
-                c = 0;
-                while (T < 2 * bound) {
-                    T <<= 1;
-                    ++c;
-                }
-            } else {
-                c = kHeavyClass;
-                atomicAdd(&cnt->heavy_flops, (unsigned long long)f);
-            }
-            int32_t pos = atomicAdd(&cnt->cls[c], 1);
-            lists[(int64_t)c * m + pos] = row;
+    int64_t f = 0;
+    if (a1 - a0 <= 32) {
+        for (int32_t p = a0; p < a1; ++p) {
+            const int32_t k = ai[p];
+            f += bp[k + 1] - bp[k];
         }
     }
+    uint64_t longm = ballot(a1 - a0 > 32);
+    while (longm) {
+        const int src = __ffsll((unsigned long long)longm) - 1;
+        longm &= longm - 1;
+        const int32_t s0 = __shfl(a0, src, kWave), s1 = __shfl(a1, src, kWave);
+        int64_t g = 0;
+        for (int32_t p = s0 + l; p < s1; p += kWave) {
+            const int32_t k = ai[p];
+            g += bp[k + 1] - bp[k];
+        }
+        g = wave_sum(g);
+        if (l == src) f = g;
+    }
+    int c = -1, lpos = 0;
+    if (row < m && f > 0) {
+        if (f <= kLightMaxF) {
+            const int64_t bound = f < ncols ? f : ncols;
+            int64_t T = 64;
+            c = 0;
+            while (T < 2 * bound) {
+                T <<= 1;
+                ++c;
+            }
+        } else {
+            c = kHeavyClass;
+        }
+        lpos = atomicAdd(&s_cnt[c], 1);
+    }
+    const unsigned long long wf = (unsigned long long)wave_sum(f);
+    const unsigned long long whf = (unsigned long long)wave_sum(c == kHeavyClass ? f : (int64_t)0);
+    if (l == 0) {
+        atomicAdd(&s_f, wf);
+        atomicAdd(&s_hf, whf);
+    }
+    __syncthreads();
+    if (tid < kNumClasses && s_cnt[tid] > 0) s_base[tid] = atomicAdd(&cnt->cls[tid], s_cnt[tid]);
+    if (tid == 0) {
+        if (s_f) atomicAdd(&cnt->flops, s_f);
+        if (s_hf) atomicAdd(&cnt->heavy_flops, s_hf);
+    }
+    __syncthreads();
+    if (c >= 0) lists[(int64_t)c * m + s_base[c] + lpos] = row;
+    if (row < m) flops[row] = f;
 }
 
 // ---------------------------------------------------------------- symbolic, light rows
@@ -199,8 +268,7 @@ __global__ __launch_bounds__(256) void k_sym_light(const int32_t *__restrict__ a
         const int nk = min(kWave, a1 - pb);
         for (int t = 0; t < nk; ++t) {
             const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
-            for (int32_t q = s + l; q < e; q += kWave) {
-                const int32_t j = bi[q];
+            stream_cols(bi, s, e, [&](int32_t j) {
                 uint32_t h = hash_slot(j, kLog);
                 for (int probe = 0; probe < T; ++probe) {
                     int32_t old = atomicCAS(&tab[h], kEmpty, j);
@@ -211,7 +279,7 @@ __global__ __launch_bounds__(256) void k_sym_light(const int32_t *__restrict__ a
                     if (old == j) break;
                     h = (h + 1) & (T - 1);
                 }
-            }
+            });
         }
     }
     cnt = wave_sum(cnt);
@@ -222,8 +290,8 @@ __global__ __launch_bounds__(256) void k_sym_light(const int32_t *__restrict__ a
 __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
     const int32_t *__restrict__ ap, const int32_t *__restrict__ ai, const int32_t *__restrict__ bp,
     const int32_t *__restrict__ bi, int32_t ncols, const int32_t *__restrict__ rows, int32_t nrows,
-    int64_t *__restrict__ rownnz, int emit, Window *__restrict__ wins, int32_t win_cap, Item *__restrict__ items,
-    int32_t item_cap, Counters *__restrict__ cnt) {
+    int64_t *__restrict__ rownnz, int emit, int32_t win_h, Window *__restrict__ wins, int32_t win_cap,
+    Item *__restrict__ items, int32_t item_cap, Counters *__restrict__ cnt) {
     constexpr int kWords = 1 << (kPassBits - 5);   // 32768 words
     constexpr int kFine = 1 << (kPassBits - kFineShift);  // 8192 fine blocks
     constexpr int kPerThread = kFine / kHeavyThreads;     // 8
@@ -245,21 +313,24 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
         const int32_t nfine = (width + (1 << kFineShift) - 1) >> kFineShift;
         for (int t = tid; t < kWords; t += kHeavyThreads) bm[t] = 0u;
         __syncthreads();
-        // mark
-        for (int32_t pb = a0 + w * kWave; pb < a1; pb += kWavesB * kWave) {
+        // mark: entry pb + w + 16 t goes to wave w, so every wave has work
+        // even when the row has only a few (long) B rows to merge
+        for (int32_t pb = a0; pb < a1; pb += kWavesB * kWave) {
             int32_t bs = 0, be = 0;
-            if (pb + l < a1) {
-                int32_t k = ai[pb + l];
+            const int32_t p = pb + w + kWavesB * l;
+            if (p < a1) {
+                int32_t k = ai[p];
                 bs = bp[k];
                 be = bp[k + 1];
             }
-            const int nk = min(kWave, a1 - pb);
+            const int32_t rem = a1 - (pb + w);
+            const int nk = rem <= 0 ? 0 : min(kWave, (rem + kWavesB - 1) / kWavesB);
             for (int t = 0; t < nk; ++t) {
                 const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
-                for (int32_t q = s + l; q < e; q += kWave) {
-                    int64_t jj = (int64_t)bi[q] - base;
+                stream_cols(bi, s, e, [&](int32_t j) {
+                    const int64_t jj = (int64_t)j - base;
                     if (jj >= 0 && jj < width) atomicOr(&bm[jj >> 5], 1u << (jj & 31));
-                }
+                });
             }
         }
         __syncthreads();
@@ -317,7 +388,9 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
         for (int u = 0; u < kPerThread; ++u) {
             const int f = tid * kPerThread + u;
             bool first = false;
-            if (f < nfine) first = (f == 0) || (fex[u] / kWinH != fex_lds[f - 1] / kWinH);
+            if (f < nfine)
+                first = (f == 0) || (fex[u] / win_h != fex_lds[f - 1] / win_h) ||
+                        ((f >> kWidthShift) != ((f - 1) >> kWidthShift));
             if (first) {
                 flags |= 1 << u;
                 ++nflag;
@@ -421,9 +494,7 @@ __global__ __launch_bounds__(256) void k_num_light(const int32_t *__restrict__ a
         for (int t = 0; t < nk; ++t) {
             const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
             const double at = __shfl(a, t, kWave);
-            for (int32_t q = s + l; q < e; q += kWave) {
-                const int32_t j = bi[q];
-                const double b = bv[q];
+            stream_entries(bi, bv, s, e, [&](int32_t j, double b) {
                 uint32_t h = hash_slot(j, kLog);
                 for (int probe = 0; probe < T; ++probe) {
                     int32_t old = atomicCAS(&keys[h], kEmpty, j);
@@ -437,7 +508,7 @@ __global__ __launch_bounds__(256) void k_num_light(const int32_t *__restrict__ a
                     }
                     h = (h + 1) & (T - 1);
                 }
-            }
+            });
         }
     }
     wave_sync();
@@ -457,17 +528,137 @@ __global__ __launch_bounds__(256) void k_num_light(const int32_t *__restrict__ a
 }
 
 // ---------------------------------------------------------------- numeric, heavy rows (windows)
+// One wave per item (a run of consecutive windows of one row).  Per window:
+// the products falling in [c0, c1) are gathered in A-order (64 A entries at
+// a time, each lane locating its B_k's sub-range through a cursor that
+// advances window by window), flattened over the lanes, hashed into an LDS
+// table and applied with fma.  Lanes of one step that hit the same slot are
+// found with a tag store/read-back and apply in lane (= A) order.  A
+// presence bitmap over the window's columns (width <= 32768) gives every
+// entry its sorted position directly (prefix popcounts), so no sort is
+// needed.  Rows with at most 64 A entries keep their A entries, B ranges and
+// cursors in registers for the whole item.
+template <int T>
+struct HeavyLds {
+    int32_t keys[T];
+    double vals[T];
+    int32_t tags[T];
+    int32_t occ[T / 2];
+    unsigned long long bm[kBmWords];
+    int32_t wpre[kBmWords];
+};
+
+// Insert and apply the products of one batch: lane l holds A entry value a
+// and the B range [s, e) of its column; products are flattened in lane order.
+template <int T>
+__device__ __forceinline__ void heavy_batch(HeavyLds<T> &L, const int32_t *__restrict__ bi,
+                                            const double *__restrict__ bv, int32_t s, int32_t e, double a, int32_t c0,
+                                            int &nocc, Counters *__restrict__ cnt) {
+    constexpr uint32_t kLog = __builtin_ctz(T);
+    const int l = lane_id();
+    const int32_t len = e - s;
+    const int32_t incl = wave_incl_scan(len);
+    const int32_t excl = incl - len;
+    const int32_t total = __shfl(incl, kWave - 1, kWave);
+    if (total == 0) return;
+    // software pipeline: the B entries of step x0 + 64 are loaded while
+    // step x0 is inserted and applied
+    int32_t jn = 0;
+    double bn = 0.0, an = 0.0;
+    {
+        const int o = owner_lane32(excl, l);
+        const int32_t so = __shfl(s, o, kWave), eo = __shfl(excl, o, kWave);
+        an = __shfl(a, o, kWave);
+        if (l < total) {
+            const int32_t q = so + (l - eo);
+            jn = bi[q];
+            bn = bv[q];
+        }
+    }
+    for (int32_t x0 = 0; x0 < total; x0 += kWave) {
+        const int32_t x = x0 + l;
+        bool act = x < total;
+        const int32_t j = jn;
+        const double b = bn, ao = an;
+        if (x0 + kWave < total) {
+            const int32_t xn = x + kWave;
+            const int o = owner_lane32(excl, xn);
+            const int32_t so = __shfl(s, o, kWave), eo = __shfl(excl, o, kWave);
+            an = __shfl(a, o, kWave);
+            if (xn < total) {
+                const int32_t q = so + (xn - eo);
+                jn = bi[q];
+                bn = bv[q];
+            }
+        }
+        uint32_t h = 0;
+        bool fresh = false;
+        if (act) {
+            h = hash_slot(j, kLog);
+            bool placed = false;
+            for (int probe = 0; probe < T; ++probe) {
+                const int32_t old = atomicCAS(&L.keys[h], kEmpty, j);
+                if (old == kEmpty) {
+                    L.vals[h] = 0.0;
+                    const int32_t bpos = j - c0;
+                    atomicOr(&L.bm[bpos >> 6], 1ull << (bpos & 63));
+                    placed = fresh = true;
+                    break;
+                }
+                if (old == j) {
+                    placed = true;
+                    break;
+                }
+                h = (h + 1) & (T - 1);
+            }
+            if (!placed) {
+                // table full: only reachable if symbolic counts are wrong
+                atomicOr(&cnt->overflow, 8);
+                act = false;
+            }
+        }
+        // record newly occupied slots (wave-local counter)
+        const uint64_t fm = ballot(fresh);
+        if (fresh) {
+            const int pos = nocc + rank_below(fm);
+            if (pos < T / 2) L.occ[pos] = (int32_t)h;  // bounded by the window size
+        }
+        nocc += __popcll(fm);
+        // lanes sharing a slot: one tag store per lane, read back
+        if (act) L.tags[h] = l;
+        wave_sync();
+        const uint64_t lose = ballot(act && L.tags[h] != l);
+        if (lose == 0ull) {
+            if (act) L.vals[h] = __fma_rn(ao, b, L.vals[h]);
+            wave_sync();
+        } else {
+            // rank each lane within its slot group (lane order = A order)
+            int rank = 0, rounds = 1;
+            uint64_t m = lose;
+            while (m) {
+                const int l0 = __ffsll((unsigned long long)m) - 1;
+                const uint32_t h0 = (uint32_t)__shfl((int)h, l0, kWave);
+                const uint64_t grp = ballot(act && h == h0);
+                if ((grp >> l) & 1ull) rank = rank_below(grp);
+                rounds = max(rounds, (int)__popcll(grp));
+                m &= ~grp;
+            }
+            for (int r = 0; r < rounds; ++r) {
+                if (act && rank == r) L.vals[h] = __fma_rn(ao, b, L.vals[h]);
+                wave_sync();
+            }
+        }
+    }
+}
+
+template <int T>
 __global__ __launch_bounds__(kWave) void k_num_heavy(
     const int32_t *__restrict__ ap, const int32_t *__restrict__ ai, const double *__restrict__ av,
     const int32_t *__restrict__ bp, const int32_t *__restrict__ bi, const double *__restrict__ bv, int32_t ncols,
     const Item *__restrict__ items, const Window *__restrict__ wins, const int64_t *__restrict__ cursor_base,
     int32_t *__restrict__ cursors, const int64_t *__restrict__ rowptr, const Panel *__restrict__ panels,
     int npanels, Counters *__restrict__ cnt) {
-    constexpr int T = kN2T;
-    constexpr uint32_t kLog = __builtin_ctz(T);
-    __shared__ int32_t keys[T];
-    __shared__ double vals[T];
-    __shared__ uint32_t tags[T];
+    __shared__ HeavyLds<T> L;
     const int l = lane_id();
     const Item I = items[blockIdx.x];
     const int32_t row = I.row;
@@ -477,102 +668,96 @@ __global__ __launch_bounds__(kWave) void k_num_heavy(
     const Panel &P = find_panel(panels, npanels, row);
     int32_t *oi_row = P.idx + (rs - P.nnz_begin);
     double *ov_row = P.val + (rs - P.nnz_begin);
+    const bool short_row = (a1 - a0) <= kWave;
 
-    for (int t = l; t < T; t += kWave) {
-        keys[t] = kEmpty;
-        tags[t] = 0xffffffffu;
+    for (int t = l; t < T; t += kWave) L.keys[t] = kEmpty;
+    for (int t = l; t < kBmWords; t += kWave) L.bm[t] = 0ull;
+    // short rows: the A entry and B range of lane l live in registers
+    int32_t rk_bs = 0, rk_be = 0, rk_cur = 0;
+    double rk_a = 0.0;
+    if (short_row && a0 + l < a1) {
+        const int32_t k = ai[a0 + l];
+        rk_a = av[a0 + l];
+        rk_bs = bp[k];
+        rk_be = bp[k + 1];
     }
     wave_sync();
+    bool have_cursor = false;
     for (int32_t wi = I.wbeg; wi < I.wend; ++wi) {
         const Window W = wins[wi];
+        if (W.cnt == 0) continue;  // no B entry of this row's A-columns lies in it
         const bool full = (W.c0 == 0 && W.c1 >= ncols);
-        const bool first = (wi == I.wbeg), last = (wi == I.wend - 1);
-        for (int32_t pb = a0; pb < a1; pb += kWave) {
-            const int32_t p = pb + l;
-            int32_t s = 0, e = 0;
-            double a = 0.0;
-            if (p < a1) {
-                const int32_t k = ai[p];
-                a = av[p];
-                const int32_t bs = bp[k], be = bp[k + 1];
-                if (full) {
-                    s = bs;
-                    e = be;
-                } else {
-                    s = first ? lower_bound_dev(bi, bs, be, W.c0) : cur[p - a0];
-                    e = gallop_dev(bi, s, be, W.c1);
-                    if (!last) cur[p - a0] = e;
-                }
+        int nocc = 0;
+        if (short_row) {
+            int32_t s = rk_bs, e = rk_be;
+            if (!full && a0 + l < a1) {
+                s = have_cursor ? rk_cur : lower_bound_dev(bi, rk_bs, rk_be, W.c0);
+                e = gallop_dev(bi, s, rk_be, W.c1);
+                rk_cur = e;
             }
-            const int32_t len = e - s;
-            const int32_t incl = wave_incl_scan(len);
-            const int32_t excl = incl - len;
-            const int32_t total = __shfl(incl, kWave - 1, kWave);
-            for (int32_t x0 = 0; x0 < total; x0 += kWave) {
-                const int32_t x = x0 + l;
-                bool act = x < total;
-                const int o = owner_lane32(excl, x);
-                const int32_t so = __shfl(s, o, kWave);
-                const int32_t eo = __shfl(excl, o, kWave);
-                const double ao = __shfl(a, o, kWave);
-                int32_t j = 0;
-                double b = 0.0;
-                uint32_t h = 0;
-                if (act) {
-                    const int32_t q = so + (x - eo);
-                    j = bi[q];
-                    b = bv[q];
-                    h = hash_slot(j, kLog);
-                    bool placed = false;
-                    for (int probe = 0; probe < T; ++probe) {
-                        int32_t old = atomicCAS(&keys[h], kEmpty, j);
-                        if (old == kEmpty) {
-                            vals[h] = 0.0;
-                            placed = true;
-                            break;
-                        }
-                        if (old == j) {
-                            placed = true;
-                            break;
-                        }
-                        h = (h + 1) & (T - 1);
-                    }
-                    if (!placed) {
-                        // table full: only reachable if symbolic counts are wrong
-                        atomicOr(&cnt->overflow, 8);
-                        act = false;
+            heavy_batch<T>(L, bi, bv, s, e, rk_a, W.c0, nocc, cnt);
+        } else {
+            const bool last = (wi == I.wend - 1);
+            for (int32_t pb = a0; pb < a1; pb += kWave) {
+                const int32_t p = pb + l;
+                int32_t s = 0, e = 0;
+                double a = 0.0;
+                if (p < a1) {
+                    const int32_t k = ai[p];
+                    a = av[p];
+                    const int32_t bs = bp[k], be = bp[k + 1];
+                    if (full) {
+                        s = bs;
+                        e = be;
+                    } else {
+                        s = have_cursor ? cur[p - a0] : lower_bound_dev(bi, bs, be, W.c0);
+                        e = gallop_dev(bi, s, be, W.c1);
+                        if (!last) cur[p - a0] = e;
                     }
                 }
-                wave_sync();
-                // apply in lane order among lanes that share a slot
-                bool pending = act;
-                while (ballot(pending)) {
-                    if (pending) atomicMin(&tags[h], (uint32_t)l);
-                    wave_sync();
-                    const bool mine = pending && tags[h] == (uint32_t)l;
-                    if (mine) vals[h] = __fma_rn(ao, b, vals[h]);
-                    wave_sync();
-                    if (mine) {
-                        tags[h] = 0xffffffffu;
-                        pending = false;
-                    }
-                    wave_sync();
-                }
+                heavy_batch<T>(L, bi, bv, s, e, a, W.c0, nocc, cnt);
+            }
+        }
+        if (!full) have_cursor = true;
+        wave_sync();
+        // prefix popcounts of the presence words (8 words per lane)
+        const int nwords = (W.c1 - W.c0 + 63) >> 6;
+        int32_t wc[kBmWords / kWave];
+        int32_t tsum = 0;
+#pragma unroll
+        for (int u = 0; u < kBmWords / kWave; ++u) {
+            const int wd = l * (kBmWords / kWave) + u;
+            wc[u] = wd < nwords ? __popcll(L.bm[wd]) : 0;
+            tsum += wc[u];
+        }
+        const int32_t tinc = wave_incl_scan(tsum);
+        {
+            int32_t run = tinc - tsum;
+#pragma unroll
+            for (int u = 0; u < kBmWords / kWave; ++u) {
+                L.wpre[l * (kBmWords / kWave) + u] = run;
+                run += wc[u];
             }
         }
         wave_sync();
-        const int n = wave_compact_lds<double>(keys, vals, T);
-        wave_sort_lds<double>(keys, vals, n);
-        if (l == 0 && n != W.cnt) atomicOr(&cnt->overflow, 4);
-        const int nw = n < W.cnt ? n : W.cnt;  // never write past the window
+        if (l == 0 && nocc != W.cnt) atomicOr(&cnt->overflow, 4);
         int32_t *oi = oi_row + W.off;
         double *ov = ov_row + W.off;
-        for (int t = l; t < nw; t += kWave) {
-            oi[t] = keys[t];
-            ov[t] = vals[t];
+        const int nlist = nocc < T / 2 ? nocc : T / 2;
+        for (int t = l; t < nlist; t += kWave) {
+            const int32_t h = L.occ[t];
+            const int32_t k = L.keys[h];
+            const int32_t bpos = k - W.c0;
+            const int wd = bpos >> 6;
+            const int32_t r = L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (bpos & 63)) - 1ull));
+            if (r < W.cnt) {  // never write past the window
+                oi[r] = k;
+                ov[r] = L.vals[h];
+            }
         }
         wave_sync();
-        for (int t = l; t < T; t += kWave) keys[t] = kEmpty;
+        for (int t = l; t < nlist; t += kWave) L.keys[L.occ[t]] = kEmpty;
+        for (int t = l; t < nwords; t += kWave) L.bm[t] = 0ull;
         wave_sync();
     }
 }
@@ -635,7 +820,7 @@ void symbolic(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, Plan &pl, bool 
     TR_HIP(hipMemsetAsync(pl.cnt, 0, sizeof(Counters), ctx->stream));
     TR_HIP(hipMemsetAsync(pl.rownnz, 0, sizeof(int64_t) * ((size_t)m + 1), ctx->stream));
     if (m > 0 && A->nnz > 0 && B->nnz > 0) {
-        TR_LAUNCH(ctx, "row_flops", k_row_flops, div_up((uint64_t)m, 4), 256, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
+        TR_LAUNCH(ctx, "row_flops", k_row_flops, div_up((uint64_t)m, kRowsPerBlock), kRowsPerBlock, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
                   m, pl.n, pl.flops, pl.lists, pl.cnt);
     }
     read_array(ctx, &pl.h, pl.cnt, sizeof(Counters));
@@ -650,9 +835,11 @@ void symbolic(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, Plan &pl, bool 
     const int32_t nh = pl.h.cls[kHeavyClass];
     if (nh) {
         if (emit) {
-            // windows per pass <= ceil(pass_nnz / kWinH) <= pass_flops / kWinH + 1
+            // windows per pass <= ceil(pass_nnz / H) + (width splits) <=
+            // pass_flops / H + 1 + pass_width / kWinMaxWidth
             const int64_t passes = ((int64_t)pl.n + (1 << kPassBits) - 1) >> kPassBits;
-            int64_t wcap = (int64_t)(pl.h.heavy_flops / kWinH) + (int64_t)nh * passes + 16;
+            const int64_t wsplits = (int64_t)pl.n / kWinMaxWidth + 1;
+            int64_t wcap = (int64_t)(pl.h.heavy_flops / kWinH) + (int64_t)nh * (2 * passes + wsplits) + 16;
             int64_t icap = wcap / kItemWindows + (int64_t)nh * passes + 16;
             TR_REQUIRE(wcap < INT32_MAX && icap < INT32_MAX, "spgemm: heavy work list exceeds int32 capacity");
             pl.win_cap = (int32_t)wcap;
@@ -661,7 +848,8 @@ void symbolic(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, Plan &pl, bool 
             pl.items = (Item *)scratch(ctx, S_ITEMS, sizeof(Item) * (size_t)icap);
         }
         TR_LAUNCH(ctx, "sym_heavy", k_sym_heavy, (uint32_t)nh, kHeavyThreads, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
-                  B->idx_vec, pl.n, L + (int64_t)kHeavyClass * M, nh, pl.rownnz, emit ? 1 : 0, pl.wins, pl.win_cap,
+                  B->idx_vec, pl.n, L + (int64_t)kHeavyClass * M, nh, pl.rownnz, emit ? 1 : 0, (int32_t)kWinH,
+                  pl.wins, pl.win_cap,
                   pl.items, pl.item_cap, pl.cnt);
     }
     launch_exclusive_scan_i64(ctx, pl.rownnz, pl.rowptr, m);
@@ -678,7 +866,7 @@ int64_t spgemm_flops(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B) {
     pl.cnt = (Counters *)scratch(ctx, S_BINCNT, sizeof(Counters));
     TR_HIP(hipMemsetAsync(pl.cnt, 0, sizeof(Counters), ctx->stream));
     if (m > 0 && A->nnz > 0 && B->nnz > 0)
-        TR_LAUNCH(ctx, "row_flops", k_row_flops, div_up((uint64_t)m, 4), 256, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
+        TR_LAUNCH(ctx, "row_flops", k_row_flops, div_up((uint64_t)m, kRowsPerBlock), kRowsPerBlock, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
                   m, B->ncols, pl.flops, pl.lists, pl.cnt);
     read_array(ctx, &pl.h, pl.cnt, sizeof(Counters));
     return (int64_t)pl.h.flops;
@@ -798,8 +986,9 @@ int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max
     if (pl.h.cls[5]) launch_num_light<2048>(ctx, A, B, L + 5 * M, pl.h.cls[5], pl.rowptr, dp, np, pl.cnt);
     if (nh && pl.h.items > 0) {
         int32_t *cursors = (int32_t *)scratch(ctx, S_CURSORS, sizeof(int32_t) * (size_t)std::max<int64_t>(ncur, 1));
-        TR_LAUNCH(ctx, "num_heavy", k_num_heavy, (uint32_t)pl.h.items, kWave, 0, A->ptr_vec, A->idx_vec, A->val,
-                  B->ptr_vec, B->idx_vec, B->val, pl.n, pl.items, pl.wins, cbase, cursors, pl.rowptr, dp, np, pl.cnt);
+        TR_LAUNCH(ctx, "num_heavy", k_num_heavy<kN2T>, (uint32_t)pl.h.items, kWave, 0, A->ptr_vec, A->idx_vec,
+                  A->val, B->ptr_vec, B->idx_vec, B->val, pl.n, pl.items, pl.wins, cbase, cursors, pl.rowptr, dp, np,
+                  pl.cnt);
     }
     // internal consistency flags (symbolic vs numeric counts)
     int32_t ov = read_scalar<int32_t>(ctx, &pl.cnt->overflow);
