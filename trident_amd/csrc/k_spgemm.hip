@@ -30,58 +30,14 @@
 // the CPU oracle and do not depend on scheduling.
 #include "dev_common.h"
 #include "tr_internal.h"
+#include "spgemm_types.h"
 
 #include <algorithm>
 #include <vector>
 
+
 namespace tr {
 
-// tuning constants (see DESIGN notes in the repository documentation)
-constexpr int kLightMaxF = 1024;   // symbolic light/heavy cut on row flops
-constexpr int kNumLight = 6;       // light classes: T = 64 << c, c < kNumLight
-constexpr int kHeavyClass = kNumLight;
-constexpr int kNumClasses = kNumLight + 1;
-constexpr int kPassBits = 20;      // heavy bitmap covers 2^20 columns per pass
-constexpr int kFineShift = 7;      // fine block = 128 columns
-constexpr int kWinH = 384;         // default window packing quantum (entries)
-constexpr int kN2T = 1024;         // default heavy window hash slots (>= 2 * (kWinH + 127))
-constexpr int kWinMaxWidth = 32768;                       // window width cap (columns)
-constexpr int kWidthShift = 15 - kFineShift;               // fine blocks per width quantum: 2^8
-constexpr int kBmWords = kWinMaxWidth / 64;                // presence words per window
-constexpr int kItemWindows = 16;   // windows per heavy work item
-constexpr int kHeavyThreads = 1024;
-
-struct Window {
-    int32_t c0, c1;   // column range [c0, c1)
-    int32_t off;      // offset of the window's first entry within the row
-    int32_t cnt;      // entries in the window
-};
-
-struct Item {
-    int32_t row;
-    int32_t wbeg, wend;  // window index range
-    int32_t pad;
-};
-
-struct Panel {
-    int64_t nnz_begin;   // global offset of the panel's first entry
-    int32_t row_begin;
-    int32_t pad;
-    int32_t *idx;
-    double *val;
-};
-
-// device counters written by the symbolic phase
-struct Counters {
-    int32_t cls[kNumClasses];        // rows per class
-    int32_t pad0;
-    unsigned long long flops;        // total multiplications
-    unsigned long long heavy_flops;  // multiplications of heavy rows
-    int32_t windows;                 // windows appended
-    int32_t items;                   // items appended
-    int32_t overflow;                // capacity exceeded
-    int32_t pad1;
-};
 
 namespace {
 
@@ -162,32 +118,63 @@ __device__ __forceinline__ void stream_entries(const int32_t *__restrict__ bi, c
     for (; q < e; q += kWave) f(bi[q], bv[q]);
 }
 
+// task owning global row g (largest t with row_base <= g)
+__device__ __forceinline__ int task_of(const TaskDev *__restrict__ tasks, int nt, int32_t g) {
+    int lo = 0, hi = nt - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tasks[mid].row_base <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ const Panel &find_panel(const Panel *__restrict__ panels, int np, int32_t row) {
-    int p = 0;
-    while (p + 1 < np && panels[p + 1].row_begin <= row) ++p;
-    return panels[p];
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (panels[mid].row_begin <= row) lo = mid;
+        else hi = mid - 1;
+    }
+    return panels[lo];
 }
 
 // ---------------------------------------------------------------- analysis
-// 256 rows per block: a thread per row for short rows, the whole wave for
-// rows with more than 32 entries; class counters are aggregated in LDS so
-// there is one global atomic per class per block.
+// 256 global rows per block: a thread per row for short rows, the whole wave
+// for rows with more than 32 entries; class counters are aggregated in LDS
+// so there is one global atomic per class per block.
 constexpr int kRowsPerBlock = 256;
 
-__global__ __launch_bounds__(kRowsPerBlock) void k_row_flops(
-    const int32_t *__restrict__ ap, const int32_t *__restrict__ ai, const int32_t *__restrict__ bp, int32_t m,
-    int32_t ncols, int64_t *__restrict__ flops, int32_t *__restrict__ lists, Counters *__restrict__ cnt) {
+__global__ void k_zero_counters(Counters *__restrict__ cnt) {
+    if (threadIdx.x < sizeof(Counters) / 4) ((int32_t *)cnt)[threadIdx.x] = 0;
+}
+
+__global__ void k_zero_i64(int64_t *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+__global__ __launch_bounds__(kRowsPerBlock) void k_row_flops(const TaskDev *__restrict__ tasks, int nt,
+                                                             int32_t rows_total, int64_t *__restrict__ flops,
+                                                             int32_t *__restrict__ lists,
+                                                             Counters *__restrict__ cnt) {
     __shared__ int32_t s_cnt[kNumClasses], s_base[kNumClasses];
     __shared__ unsigned long long s_f, s_hf;
     const int tid = threadIdx.x, l = lane_id();
-    const int32_t row = blockIdx.x * kRowsPerBlock + tid;
+    const int32_t g = blockIdx.x * kRowsPerBlock + tid;
     if (tid < kNumClasses) s_cnt[tid] = 0;
     if (tid == 0) s_f = s_hf = 0ull;
     __syncthreads();
-    int32_t a0 = 0, a1 = 0;
-    if (row < m) {
-        a0 = ap[row];
-        a1 = ap[row + 1];
+    int32_t a0 = 0, a1 = 0, ncols = 0;
+    const int32_t *ai = nullptr, *bp = nullptr;
+    if (g < rows_total) {
+        const TaskDev &T = tasks[task_of(tasks, nt, g)];
+        const int32_t r = g - T.row_base;
+        a0 = T.ap[r];
+        a1 = T.ap[r + 1];
+        ai = T.ai;
+        bp = T.bp;
+        ncols = T.n;
     }
     int64_t f = 0;
     if (a1 - a0 <= 32) {
@@ -201,16 +188,18 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_row_flops(
         const int src = __ffsll((unsigned long long)longm) - 1;
         longm &= longm - 1;
         const int32_t s0 = __shfl(a0, src, kWave), s1 = __shfl(a1, src, kWave);
-        int64_t g = 0;
+        const int32_t *sai = (const int32_t *)__shfl((long long)ai, src, kWave);
+        const int32_t *sbp = (const int32_t *)__shfl((long long)bp, src, kWave);
+        int64_t acc = 0;
         for (int32_t p = s0 + l; p < s1; p += kWave) {
-            const int32_t k = ai[p];
-            g += bp[k + 1] - bp[k];
+            const int32_t k = sai[p];
+            acc += sbp[k + 1] - sbp[k];
         }
-        g = wave_sum(g);
-        if (l == src) f = g;
+        acc = wave_sum(acc);
+        if (l == src) f = acc;
     }
     int c = -1, lpos = 0;
-    if (row < m && f > 0) {
+    if (g < rows_total && f > 0) {
         if (f <= kLightMaxF) {
             const int64_t bound = f < ncols ? f : ncols;
             int64_t T = 64;
@@ -237,62 +226,73 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_row_flops(
         if (s_hf) atomicAdd(&cnt->heavy_flops, s_hf);
     }
     __syncthreads();
-    if (c >= 0) lists[(int64_t)c * m + s_base[c] + lpos] = row;
-    if (row < m) flops[row] = f;
+    if (c >= 0) lists[(int64_t)c * rows_total + s_base[c] + lpos] = g;
+    if (g < rows_total) flops[g] = f;
 }
 
 // ---------------------------------------------------------------- symbolic, light rows
+// One wave per row, rows taken from the class list (grid-stride, count read
+// on the device so the launch can be replayed from a graph).
 template <int T>
-__global__ __launch_bounds__(256) void k_sym_light(const int32_t *__restrict__ ap, const int32_t *__restrict__ ai,
-                                                   const int32_t *__restrict__ bp, const int32_t *__restrict__ bi,
-                                                   const int32_t *__restrict__ rows, int32_t nrows,
-                                                   int64_t *__restrict__ rownnz) {
+__global__ __launch_bounds__(256) void k_sym_light(const TaskDev *__restrict__ tasks, int nt,
+                                                   const int32_t *__restrict__ rows, const Counters *__restrict__ cnt,
+                                                   int cls, int64_t *__restrict__ rownnz) {
     __shared__ int32_t tabs[4][T];
     constexpr uint32_t kLog = __builtin_ctz(T);
     const int w = threadIdx.x / kWave, l = lane_id();
-    const int32_t li = blockIdx.x * 4 + w;
-    if (li >= nrows) return;
     int32_t *tab = tabs[w];
-    const int32_t row = rows[li];
-    for (int t = l; t < T; t += kWave) tab[t] = kEmpty;
-    wave_sync();
-    const int32_t a0 = ap[row], a1 = ap[row + 1];
-    int cnt = 0;
-    for (int32_t pb = a0; pb < a1; pb += kWave) {
-        int32_t bs = 0, be = 0;
-        if (pb + l < a1) {
-            int32_t k = ai[pb + l];
-            bs = bp[k];
-            be = bp[k + 1];
-        }
-        const int nk = min(kWave, a1 - pb);
-        for (int t = 0; t < nk; ++t) {
-            const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
-            stream_cols(bi, s, e, [&](int32_t j) {
-                uint32_t h = hash_slot(j, kLog);
-                for (int probe = 0; probe < T; ++probe) {
-                    int32_t old = atomicCAS(&tab[h], kEmpty, j);
-                    if (old == kEmpty) {
-                        ++cnt;
-                        break;
+    const int32_t nrows = cnt->cls[cls];
+    for (int32_t li = blockIdx.x * 4 + w; li < nrows; li += gridDim.x * 4) {
+        const int32_t g = rows[li];
+        const TaskDev &TK = tasks[task_of(tasks, nt, g)];
+        const int32_t r = g - TK.row_base;
+        const int32_t *__restrict__ ai = TK.ai;
+        const int32_t *__restrict__ bp = TK.bp;
+        const int32_t *__restrict__ bi = TK.bi;
+        for (int t = l; t < T; t += kWave) tab[t] = kEmpty;
+        wave_sync();
+        const int32_t a0 = TK.ap[r], a1 = TK.ap[r + 1];
+        int cntv = 0;
+        for (int32_t pb = a0; pb < a1; pb += kWave) {
+            int32_t bs = 0, be = 0;
+            if (pb + l < a1) {
+                const int32_t k = ai[pb + l];
+                bs = bp[k];
+                be = bp[k + 1];
+            }
+            const int nk = min(kWave, a1 - pb);
+            for (int t = 0; t < nk; ++t) {
+                const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
+                stream_cols(bi, s, e, [&](int32_t j) {
+                    uint32_t h = hash_slot(j, kLog);
+                    for (int probe = 0; probe < T; ++probe) {
+                        const int32_t old = atomicCAS(&tab[h], kEmpty, j);
+                        if (old == kEmpty) {
+                            ++cntv;
+                            break;
+                        }
+                        if (old == j) break;
+                        h = (h + 1) & (T - 1);
                     }
-                    if (old == j) break;
-                    h = (h + 1) & (T - 1);
-                }
-            });
+                });
+            }
         }
+        cntv = wave_sum(cntv);
+        if (l == 0) rownnz[g] = cntv;
+        wave_sync();
     }
-    cnt = wave_sum(cnt);
-    if (l == 0) rownnz[row] = cnt;
 }
 
 // ---------------------------------------------------------------- symbolic, heavy rows
+// One 1024-thread block per row (grid-stride over the heavy list): a bitmap
+// of 2^20 columns in LDS per pass, popcounts per 128-column fine block, and
+// the row's column windows (<= win_h + 127 entries, <= kWinMaxWidth columns)
+// appended to the device work lists.
 __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
-    const int32_t *__restrict__ ap, const int32_t *__restrict__ ai, const int32_t *__restrict__ bp,
-    const int32_t *__restrict__ bi, int32_t ncols, const int32_t *__restrict__ rows, int32_t nrows,
+    const TaskDev *__restrict__ tasks, int nt, const int32_t *__restrict__ rows, Counters *__restrict__ cnt,
     int64_t *__restrict__ rownnz, int emit, int32_t win_h, Window *__restrict__ wins, int32_t win_cap,
-    Item *__restrict__ items, int32_t item_cap, Counters *__restrict__ cnt) {
-    constexpr int kWords = 1 << (kPassBits - 5);   // 32768 words
+    Item *__restrict__ items, int32_t item_cap) {
+    constexpr int kWords = 1 << (kPassBits - 5);          // 32768 words
     constexpr int kFine = 1 << (kPassBits - kFineShift);  // 8192 fine blocks
     constexpr int kPerThread = kFine / kHeavyThreads;     // 8
     constexpr int kWavesB = kHeavyThreads / kWave;        // 16
@@ -301,249 +301,257 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
     __shared__ int32_t sh_wbase;
 
     const int tid = threadIdx.x, w = tid / kWave, l = lane_id();
-    const int32_t row = rows[blockIdx.x];
-    const int32_t a0 = ap[row], a1 = ap[row + 1];
-    int32_t row_total = 0;
-    int32_t *fex_lds = (int32_t *)bm;              // [0, kFine) after marking
-    int32_t *first_lds = (int32_t *)bm + kFine;    // [kFine, 2 kFine)
-
-    for (int64_t base = 0; base < ncols; base += (1 << kPassBits)) {
-        const int32_t width = (int32_t)min<int64_t>(1 << kPassBits, (int64_t)ncols - base);
-        const int32_t nwords = (width + 31) >> 5;
-        const int32_t nfine = (width + (1 << kFineShift) - 1) >> kFineShift;
-        for (int t = tid; t < kWords; t += kHeavyThreads) bm[t] = 0u;
-        __syncthreads();
-        // mark: entry pb + w + 16 t goes to wave w, so every wave has work
-        // even when the row has only a few (long) B rows to merge
-        for (int32_t pb = a0; pb < a1; pb += kWavesB * kWave) {
-            int32_t bs = 0, be = 0;
-            const int32_t p = pb + w + kWavesB * l;
-            if (p < a1) {
-                int32_t k = ai[p];
-                bs = bp[k];
-                be = bp[k + 1];
-            }
-            const int32_t rem = a1 - (pb + w);
-            const int nk = rem <= 0 ? 0 : min(kWave, (rem + kWavesB - 1) / kWavesB);
-            for (int t = 0; t < nk; ++t) {
-                const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
-                stream_cols(bi, s, e, [&](int32_t j) {
-                    const int64_t jj = (int64_t)j - base;
-                    if (jj >= 0 && jj < width) atomicOr(&bm[jj >> 5], 1u << (jj & 31));
-                });
-            }
-        }
-        __syncthreads();
-        // fine-block counts, 8 consecutive fine blocks per thread
-        int32_t c[kPerThread];
-        int32_t tsum = 0;
-#pragma unroll
-        for (int u = 0; u < kPerThread; ++u) {
-            const int f = tid * kPerThread + u;
-            int32_t v = 0;
-            if (f < nfine) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    int wd = f * 4 + q;
-                    if (wd < nwords) v += __popc(bm[wd]);
+    int32_t *fex_lds = (int32_t *)bm;            // [0, kFine) after marking
+    int32_t *first_lds = (int32_t *)bm + kFine;  // [kFine, 2 kFine)
+    const int32_t nheavy = cnt->cls[kHeavyClass];
+    for (int32_t hi = blockIdx.x; hi < nheavy; hi += gridDim.x) {
+        const int32_t g = rows[hi];
+        const TaskDev &TK = tasks[task_of(tasks, nt, g)];
+        const int32_t r = g - TK.row_base;
+        const int32_t *__restrict__ ai = TK.ai;
+        const int32_t *__restrict__ bp = TK.bp;
+        const int32_t *__restrict__ bi = TK.bi;
+        const int32_t ncols = TK.n;
+        const int32_t a0 = TK.ap[r], a1 = TK.ap[r + 1];
+        int32_t row_total = 0;
+        for (int64_t base = 0; base < ncols; base += (1 << kPassBits)) {
+            const int32_t width = (int32_t)min<int64_t>(1 << kPassBits, (int64_t)ncols - base);
+            const int32_t nwords = (width + 31) >> 5;
+            const int32_t nfine = (width + (1 << kFineShift) - 1) >> kFineShift;
+            for (int t = tid; t < kWords; t += kHeavyThreads) bm[t] = 0u;
+            __syncthreads();
+            // mark: entry pb + w + 16 t goes to wave w
+            for (int32_t pb = a0; pb < a1; pb += kWavesB * kWave) {
+                int32_t bs = 0, be = 0;
+                const int32_t p = pb + w + kWavesB * l;
+                if (p < a1) {
+                    const int32_t k = ai[p];
+                    bs = bp[k];
+                    be = bp[k + 1];
+                }
+                const int32_t rem = a1 - (pb + w);
+                const int nk = rem <= 0 ? 0 : min(kWave, (rem + kWavesB - 1) / kWavesB);
+                for (int t = 0; t < nk; ++t) {
+                    const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
+                    stream_cols(bi, s, e, [&](int32_t j) {
+                        const int64_t jj = (int64_t)j - base;
+                        if (jj >= 0 && jj < width) atomicOr(&bm[jj >> 5], 1u << (jj & 31));
+                    });
                 }
             }
-            c[u] = v;
-            tsum += v;
-        }
-        // block exclusive scan of tsum
-        int32_t inc = wave_incl_scan(tsum);
-        if (l == kWave - 1) wsum[w] = inc;
-        __syncthreads();
-        int32_t off = 0, pass_total = 0;
-        for (int v = 0; v < kWavesB; ++v) {
-            if (v < w) off += wsum[v];
-            pass_total += wsum[v];
-        }
-        int32_t fex[kPerThread];
-        {
-            int32_t run = off + inc - tsum;
+            __syncthreads();
+            // fine-block counts, 8 consecutive fine blocks per thread
+            int32_t c[kPerThread];
+            int32_t tsum = 0;
 #pragma unroll
             for (int u = 0; u < kPerThread; ++u) {
-                fex[u] = run;
-                run += c[u];
+                const int f = tid * kPerThread + u;
+                int32_t v = 0;
+                if (f < nfine) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int wd = f * 4 + q;
+                        if (wd < nwords) v += __popc(bm[wd]);
+                    }
+                }
+                c[u] = v;
+                tsum += v;
             }
-        }
-        if (pass_total == 0) {
+            const int32_t inc = wave_incl_scan(tsum);
+            if (l == kWave - 1) wsum[w] = inc;
             __syncthreads();
-            continue;
-        }
-        if (!emit) {
-            row_total += pass_total;
-            __syncthreads();
-            continue;
-        }
-        // fine prefixes to LDS (bitmap no longer needed)
-#pragma unroll
-        for (int u = 0; u < kPerThread; ++u) fex_lds[tid * kPerThread + u] = fex[u];
-        __syncthreads();
-        // window starts: first(f) = f == 0 || fex[f]/H != fex[f-1]/H
-        int flags = 0, nflag = 0;
-#pragma unroll
-        for (int u = 0; u < kPerThread; ++u) {
-            const int f = tid * kPerThread + u;
-            bool first = false;
-            if (f < nfine)
-                first = (f == 0) || (fex[u] / win_h != fex_lds[f - 1] / win_h) ||
-                        ((f >> kWidthShift) != ((f - 1) >> kWidthShift));
-            if (first) {
-                flags |= 1 << u;
-                ++nflag;
+            int32_t off = 0, pass_total = 0;
+            for (int v = 0; v < kWavesB; ++v) {
+                if (v < w) off += wsum[v];
+                pass_total += wsum[v];
             }
-        }
-        int32_t finc = wave_incl_scan(nflag);
-        __syncthreads();
-        if (l == kWave - 1) wsum[w] = finc;
-        __syncthreads();
-        int32_t foff = 0, nwin = 0;
-        for (int v = 0; v < kWavesB; ++v) {
-            if (v < w) foff += wsum[v];
-            nwin += wsum[v];
-        }
-        int32_t r = foff + finc - nflag;
+            int32_t fex[kPerThread];
+            {
+                int32_t run = off + inc - tsum;
 #pragma unroll
-        for (int u = 0; u < kPerThread; ++u)
-            if (flags & (1 << u)) first_lds[r++] = tid * kPerThread + u;
-        if (tid == 0) {
-            int32_t nitems = (nwin + kItemWindows - 1) / kItemWindows;
-            int32_t wb = atomicAdd(&cnt->windows, nwin);
-            int32_t ib = atomicAdd(&cnt->items, nitems);
-            if (wb + nwin > win_cap || ib + nitems > item_cap) {
-                atomicOr(&cnt->overflow, 1);
-                sh_wbase = -1;
-            } else {
-                sh_wbase = wb;
-                for (int32_t it = 0; it < nitems; ++it) {
-                    Item I;
-                    I.row = row;
-                    I.wbeg = wb + it * kItemWindows;
-                    I.wend = min(wb + nwin, I.wbeg + kItemWindows);
-                    I.pad = 0;
-                    items[ib + it] = I;
+                for (int u = 0; u < kPerThread; ++u) {
+                    fex[u] = run;
+                    run += c[u];
                 }
             }
-        }
-        __syncthreads();
-        const int32_t wbase = sh_wbase;
-        if (wbase >= 0) {
-            for (int32_t rr = tid; rr < nwin; rr += kHeavyThreads) {
-                const int32_t f = first_lds[rr];
-                const int32_t fn = (rr + 1 < nwin) ? first_lds[rr + 1] : nfine;
-                Window W;
-                W.c0 = (int32_t)(base + ((int64_t)f << kFineShift));
-                W.c1 = (int32_t)min<int64_t>(base + ((int64_t)fn << kFineShift), base + width);
-                W.off = row_total + fex_lds[f];
-                W.cnt = (fn < nfine ? fex_lds[fn] : pass_total) - fex_lds[f];
-                wins[wbase + rr] = W;
+            if (pass_total == 0 || !emit) {
+                row_total += pass_total;
+                __syncthreads();
+                continue;
             }
+#pragma unroll
+            for (int u = 0; u < kPerThread; ++u) fex_lds[tid * kPerThread + u] = fex[u];
+            __syncthreads();
+            // window starts: a count quantum or a width quantum begins
+            int flags = 0, nflag = 0;
+#pragma unroll
+            for (int u = 0; u < kPerThread; ++u) {
+                const int f = tid * kPerThread + u;
+                bool first = false;
+                if (f < nfine)
+                    first = (f == 0) || (fex[u] / win_h != fex_lds[f - 1] / win_h) ||
+                            ((f >> kWidthShift) != ((f - 1) >> kWidthShift));
+                if (first) {
+                    flags |= 1 << u;
+                    ++nflag;
+                }
+            }
+            const int32_t finc = wave_incl_scan(nflag);
+            __syncthreads();
+            if (l == kWave - 1) wsum[w] = finc;
+            __syncthreads();
+            int32_t foff = 0, nwin = 0;
+            for (int v = 0; v < kWavesB; ++v) {
+                if (v < w) foff += wsum[v];
+                nwin += wsum[v];
+            }
+            int32_t rk = foff + finc - nflag;
+#pragma unroll
+            for (int u = 0; u < kPerThread; ++u)
+                if (flags & (1 << u)) first_lds[rk++] = tid * kPerThread + u;
+            if (tid == 0) {
+                const int32_t nitems = (nwin + kItemWindows - 1) / kItemWindows;
+                const int32_t wb = atomicAdd(&cnt->windows, nwin);
+                const int32_t ib = atomicAdd(&cnt->items, nitems);
+                if (wb + nwin > win_cap || ib + nitems > item_cap) {
+                    atomicOr(&cnt->overflow, 1);
+                    sh_wbase = -1;
+                } else {
+                    sh_wbase = wb;
+                    for (int32_t it = 0; it < nitems; ++it) {
+                        Item I;
+                        I.row = g;
+                        I.wbeg = wb + it * kItemWindows;
+                        I.wend = min(wb + nwin, I.wbeg + kItemWindows);
+                        I.pad = 0;
+                        items[ib + it] = I;
+                    }
+                }
+            }
+            __syncthreads();
+            const int32_t wbase = sh_wbase;
+            if (wbase >= 0) {
+                for (int32_t rr = tid; rr < nwin; rr += kHeavyThreads) {
+                    const int32_t f = first_lds[rr];
+                    const int32_t fn = (rr + 1 < nwin) ? first_lds[rr + 1] : nfine;
+                    Window W;
+                    W.c0 = (int32_t)(base + ((int64_t)f << kFineShift));
+                    W.c1 = (int32_t)min<int64_t>(base + ((int64_t)fn << kFineShift), base + width);
+                    W.off = row_total + fex_lds[f];
+                    W.cnt = (fn < nfine ? fex_lds[fn] : pass_total) - fex_lds[f];
+                    wins[wbase + rr] = W;
+                }
+            }
+            row_total += pass_total;
+            __syncthreads();
         }
-        row_total += pass_total;
+        if (tid == 0) rownnz[g] = row_total;
         __syncthreads();
     }
-    if (tid == 0) rownnz[row] = row_total;
 }
 
-// per item: number of cursor slots it needs (multi-window items only)
+// per item: cursor slots it needs (multi-window items only)
 __global__ void k_item_cursor_len(const Item *__restrict__ items, const Counters *__restrict__ cnt,
-                                  const int32_t *__restrict__ ap, int32_t cap, int32_t *__restrict__ len) {
-    int32_t it = blockIdx.x * blockDim.x + threadIdx.x;
+                                  const TaskDev *__restrict__ tasks, int nt, int32_t cap,
+                                  int32_t *__restrict__ len) {
+    const int32_t it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= cap) return;
     int32_t v = 0;
     if (it < cnt->items) {
-        Item I = items[it];
-        if (I.wend - I.wbeg > 1) v = ap[I.row + 1] - ap[I.row];
+        const Item I = items[it];
+        if (I.wend - I.wbeg > 1) {
+            const TaskDev &TK = tasks[task_of(tasks, nt, I.row)];
+            const int32_t r = I.row - TK.row_base;
+            v = TK.ap[r + 1] - TK.ap[r];
+        }
     }
     len[it] = v;
 }
 
+// panel p starts at global row row_begin: its first entry's global offset
+__global__ void k_panel_bases(Panel *__restrict__ panels, int np, const int64_t *__restrict__ rowptr) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) panels[p].nnz_begin = rowptr[panels[p].row_begin];
+}
+
 // ---------------------------------------------------------------- numeric, light rows
 template <int T>
-__global__ __launch_bounds__(256) void k_num_light(const int32_t *__restrict__ ap, const int32_t *__restrict__ ai,
-                                                   const double *__restrict__ av, const int32_t *__restrict__ bp,
-                                                   const int32_t *__restrict__ bi, const double *__restrict__ bv,
-                                                   const int32_t *__restrict__ rows, int32_t nrows,
-                                                   const int64_t *__restrict__ rowptr, const Panel *__restrict__ panels,
-                                                   int npanels, Counters *__restrict__ cnt) {
+__global__ __launch_bounds__(256) void k_num_light(const TaskDev *__restrict__ tasks, int nt,
+                                                   const int32_t *__restrict__ rows, Counters *__restrict__ cnt,
+                                                   int cls, const int64_t *__restrict__ rowptr,
+                                                   const Panel *__restrict__ panels, int npanels) {
     __shared__ int32_t keys_s[4][T];
     __shared__ double vals_s[4][T];
     constexpr uint32_t kLog = __builtin_ctz(T);
     const int w = threadIdx.x / kWave, l = lane_id();
-    const int32_t li = blockIdx.x * 4 + w;
-    if (li >= nrows) return;
     int32_t *keys = keys_s[w];
     double *vals = vals_s[w];
-    const int32_t row = rows[li];
-    for (int t = l; t < T; t += kWave) keys[t] = kEmpty;
-    wave_sync();
-    const int32_t a0 = ap[row], a1 = ap[row + 1];
-    for (int32_t pb = a0; pb < a1; pb += kWave) {
-        int32_t bs = 0, be = 0;
-        double a = 0.0;
-        if (pb + l < a1) {
-            int32_t k = ai[pb + l];
-            a = av[pb + l];
-            bs = bp[k];
-            be = bp[k + 1];
-        }
-        const int nk = min(kWave, a1 - pb);
-        for (int t = 0; t < nk; ++t) {
-            const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
-            const double at = __shfl(a, t, kWave);
-            stream_entries(bi, bv, s, e, [&](int32_t j, double b) {
-                uint32_t h = hash_slot(j, kLog);
-                for (int probe = 0; probe < T; ++probe) {
-                    int32_t old = atomicCAS(&keys[h], kEmpty, j);
-                    if (old == kEmpty) {
-                        vals[h] = __fma_rn(at, b, 0.0);
-                        break;
+    const int32_t nrows = cnt->cls[cls];
+    for (int32_t li = blockIdx.x * 4 + w; li < nrows; li += gridDim.x * 4) {
+        const int32_t g = rows[li];
+        const TaskDev &TK = tasks[task_of(tasks, nt, g)];
+        const int32_t r = g - TK.row_base;
+        const int32_t *__restrict__ ai = TK.ai;
+        const double *__restrict__ av = TK.av;
+        const int32_t *__restrict__ bp = TK.bp;
+        const int32_t *__restrict__ bi = TK.bi;
+        const double *__restrict__ bv = TK.bv;
+        for (int t = l; t < T; t += kWave) keys[t] = kEmpty;
+        wave_sync();
+        const int32_t a0 = TK.ap[r], a1 = TK.ap[r + 1];
+        for (int32_t pb = a0; pb < a1; pb += kWave) {
+            int32_t bs = 0, be = 0;
+            double a = 0.0;
+            if (pb + l < a1) {
+                const int32_t k = ai[pb + l];
+                a = av[pb + l];
+                bs = bp[k];
+                be = bp[k + 1];
+            }
+            const int nk = min(kWave, a1 - pb);
+            for (int t = 0; t < nk; ++t) {
+                const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
+                const double at = __shfl(a, t, kWave);
+                stream_entries(bi, bv, s, e, [&](int32_t j, double b) {
+                    uint32_t h = hash_slot(j, kLog);
+                    for (int probe = 0; probe < T; ++probe) {
+                        const int32_t old = atomicCAS(&keys[h], kEmpty, j);
+                        if (old == kEmpty) {
+                            vals[h] = __fma_rn(at, b, 0.0);
+                            break;
+                        }
+                        if (old == j) {
+                            vals[h] = __fma_rn(at, b, vals[h]);
+                            break;
+                        }
+                        h = (h + 1) & (T - 1);
                     }
-                    if (old == j) {
-                        vals[h] = __fma_rn(at, b, vals[h]);
-                        break;
-                    }
-                    h = (h + 1) & (T - 1);
-                }
-            });
+                });
+            }
         }
-    }
-    wave_sync();
-    const int n = wave_compact_lds<double>(keys, vals, T);
-    wave_sort_lds<double>(keys, vals, n);
-    const int64_t rs = rowptr[row];
-    const int expect = (int)(rowptr[row + 1] - rs);
-    if (l == 0 && n != expect) atomicOr(&cnt->overflow, 2);
-    const int nw = n < expect ? n : expect;  // never write past the row
-    const Panel &P = find_panel(panels, npanels, row);
-    int32_t *oi = P.idx + (rs - P.nnz_begin);
-    double *ov = P.val + (rs - P.nnz_begin);
-    for (int t = l; t < nw; t += kWave) {
-        oi[t] = keys[t];
-        ov[t] = vals[t];
+        wave_sync();
+        const int n = wave_compact_lds<double>(keys, vals, T);
+        wave_sort_lds<double>(keys, vals, n);
+        const int64_t rs = rowptr[g];
+        const int expect = (int)(rowptr[g + 1] - rs);
+        if (l == 0 && n != expect) atomicOr(&cnt->overflow, 2);
+        const int nw = n < expect ? n : expect;  // never write past the row
+        const Panel &P = find_panel(panels, npanels, g);
+        int32_t *oi = P.idx + (rs - P.nnz_begin);
+        double *ov = P.val + (rs - P.nnz_begin);
+        for (int t = l; t < nw; t += kWave) {
+            oi[t] = keys[t];
+            ov[t] = vals[t];
+        }
+        wave_sync();
     }
 }
 
-// ---------------------------------------------------------------- numeric, heavy rows (windows)
-// One wave per item (a run of consecutive windows of one row).  Per window:
-// the products falling in [c0, c1) are gathered in A-order (64 A entries at
-// a time, each lane locating its B_k's sub-range through a cursor that
-// advances window by window), flattened over the lanes, hashed into an LDS
-// table and applied with fma.  Lanes of one step that hit the same slot are
-// found with a tag store/read-back and apply in lane (= A) order.  A
-// presence bitmap over the window's columns (width <= 32768) gives every
-// entry its sorted position directly (prefix popcounts), so no sort is
-// needed.  Rows with at most 64 A entries keep their A entries, B ranges and
-// cursors in registers for the whole item.
 template <int T>
 struct HeavyLds {
     int32_t keys[T];
     double vals[T];
     int32_t tags[T];
-    int32_t occ[T / 2];
     unsigned long long bm[kBmWords];
     int32_t wpre[kBmWords];
 };
@@ -617,185 +625,323 @@ __device__ __forceinline__ void heavy_batch(HeavyLds<T> &L, const int32_t *__res
                 act = false;
             }
         }
-        // record newly occupied slots (wave-local counter)
-        const uint64_t fm = ballot(fresh);
-        if (fresh) {
-            const int pos = nocc + rank_below(fm);
-            if (pos < T / 2) L.occ[pos] = (int32_t)h;  // bounded by the window size
-        }
-        nocc += __popcll(fm);
-        // lanes sharing a slot: one tag store per lane, read back
-        if (act) L.tags[h] = l;
+        // count newly occupied slots (wave-local counter)
+        nocc += __popcll(ballot(fresh));
         wave_sync();
-        const uint64_t lose = ballot(act && L.tags[h] != l);
-        if (lose == 0ull) {
-            if (act) L.vals[h] = __fma_rn(ao, b, L.vals[h]);
+        bool pending = act;
+        while (ballot(pending)) {
+            if (pending) atomicMin((uint32_t *)&L.tags[h], (uint32_t)l);
             wave_sync();
-        } else {
-            // rank each lane within its slot group (lane order = A order)
-            int rank = 0, rounds = 1;
-            uint64_t m = lose;
-            while (m) {
-                const int l0 = __ffsll((unsigned long long)m) - 1;
-                const uint32_t h0 = (uint32_t)__shfl((int)h, l0, kWave);
-                const uint64_t grp = ballot(act && h == h0);
-                if ((grp >> l) & 1ull) rank = rank_below(grp);
-                rounds = max(rounds, (int)__popcll(grp));
-                m &= ~grp;
+            const bool mine = pending && L.tags[h] == l;
+            if (mine) L.vals[h] = __fma_rn(ao, b, L.vals[h]);
+            wave_sync();
+            if (mine) {
+                L.tags[h] = -1;
+                pending = false;
             }
-            for (int r = 0; r < rounds; ++r) {
-                if (act && rank == r) L.vals[h] = __fma_rn(ao, b, L.vals[h]);
-                wave_sync();
-            }
+            wave_sync();
         }
     }
 }
 
 template <int T>
-__global__ __launch_bounds__(kWave) void k_num_heavy(
-    const int32_t *__restrict__ ap, const int32_t *__restrict__ ai, const double *__restrict__ av,
-    const int32_t *__restrict__ bp, const int32_t *__restrict__ bi, const double *__restrict__ bv, int32_t ncols,
-    const Item *__restrict__ items, const Window *__restrict__ wins, const int64_t *__restrict__ cursor_base,
-    int32_t *__restrict__ cursors, const int64_t *__restrict__ rowptr, const Panel *__restrict__ panels,
-    int npanels, Counters *__restrict__ cnt) {
+__global__ __launch_bounds__(kWave) void k_num_heavy(const TaskDev *__restrict__ tasks, int nt,
+                                                     const Item *__restrict__ items, Counters *__restrict__ cnt,
+                                                     const Window *__restrict__ wins,
+                                                     const int64_t *__restrict__ cursor_base,
+                                                     int32_t *__restrict__ cursors, const int64_t *__restrict__ rowptr,
+                                                     const Panel *__restrict__ panels, int npanels) {
     __shared__ HeavyLds<T> L;
     const int l = lane_id();
-    const Item I = items[blockIdx.x];
-    const int32_t row = I.row;
-    const int32_t a0 = ap[row], a1 = ap[row + 1];
-    int32_t *cur = cursors + cursor_base[blockIdx.x];
-    const int64_t rs = rowptr[row];
-    const Panel &P = find_panel(panels, npanels, row);
-    int32_t *oi_row = P.idx + (rs - P.nnz_begin);
-    double *ov_row = P.val + (rs - P.nnz_begin);
-    const bool short_row = (a1 - a0) <= kWave;
-
-    for (int t = l; t < T; t += kWave) L.keys[t] = kEmpty;
-    for (int t = l; t < kBmWords; t += kWave) L.bm[t] = 0ull;
-    // short rows: the A entry and B range of lane l live in registers
-    int32_t rk_bs = 0, rk_be = 0, rk_cur = 0;
-    double rk_a = 0.0;
-    if (short_row && a0 + l < a1) {
-        const int32_t k = ai[a0 + l];
-        rk_a = av[a0 + l];
-        rk_bs = bp[k];
-        rk_be = bp[k + 1];
+    for (int t = l; t < T; t += kWave) {
+        L.keys[t] = kEmpty;
+        L.tags[t] = -1;
     }
+    for (int t = l; t < kBmWords; t += kWave) L.bm[t] = 0ull;
     wave_sync();
-    bool have_cursor = false;
-    for (int32_t wi = I.wbeg; wi < I.wend; ++wi) {
-        const Window W = wins[wi];
-        if (W.cnt == 0) continue;  // no B entry of this row's A-columns lies in it
-        const bool full = (W.c0 == 0 && W.c1 >= ncols);
-        int nocc = 0;
-        if (short_row) {
-            int32_t s = rk_bs, e = rk_be;
-            if (!full && a0 + l < a1) {
-                s = have_cursor ? rk_cur : lower_bound_dev(bi, rk_bs, rk_be, W.c0);
-                e = gallop_dev(bi, s, rk_be, W.c1);
-                rk_cur = e;
-            }
-            heavy_batch<T>(L, bi, bv, s, e, rk_a, W.c0, nocc, cnt);
-        } else {
-            const bool last = (wi == I.wend - 1);
-            for (int32_t pb = a0; pb < a1; pb += kWave) {
-                const int32_t p = pb + l;
-                int32_t s = 0, e = 0;
-                double a = 0.0;
-                if (p < a1) {
-                    const int32_t k = ai[p];
-                    a = av[p];
-                    const int32_t bs = bp[k], be = bp[k + 1];
-                    if (full) {
-                        s = bs;
-                        e = be;
-                    } else {
-                        s = have_cursor ? cur[p - a0] : lower_bound_dev(bi, bs, be, W.c0);
-                        e = gallop_dev(bi, s, be, W.c1);
-                        if (!last) cur[p - a0] = e;
-                    }
+    const int32_t nitems = cnt->items;
+    for (int32_t itx = blockIdx.x; itx < nitems; itx += gridDim.x) {
+        const Item I = items[itx];
+        const int32_t g = I.row;
+        const TaskDev &TK = tasks[task_of(tasks, nt, g)];
+        const int32_t r = g - TK.row_base;
+        const int32_t *__restrict__ ai = TK.ai;
+        const double *__restrict__ av = TK.av;
+        const int32_t *__restrict__ bp = TK.bp;
+        const int32_t *__restrict__ bi = TK.bi;
+        const double *__restrict__ bv = TK.bv;
+        const int32_t ncols = TK.n;
+        const int32_t a0 = TK.ap[r], a1 = TK.ap[r + 1];
+        int32_t *cur = cursors + cursor_base[itx];
+        const int64_t rs = rowptr[g];
+        const Panel &P = find_panel(panels, npanels, g);
+        int32_t *oi_row = P.idx + (rs - P.nnz_begin);
+        double *ov_row = P.val + (rs - P.nnz_begin);
+        const bool short_row = (a1 - a0) <= kWave;
+        // short rows: the A entry and B range of lane l live in registers
+        int32_t rk_bs = 0, rk_be = 0, rk_cur = 0;
+        double rk_a = 0.0;
+        if (short_row && a0 + l < a1) {
+            const int32_t k = ai[a0 + l];
+            rk_a = av[a0 + l];
+            rk_bs = bp[k];
+            rk_be = bp[k + 1];
+        }
+        bool have_cursor = false;
+        for (int32_t wi = I.wbeg; wi < I.wend; ++wi) {
+            const Window W = wins[wi];
+            if (W.cnt == 0) continue;  // no B entry of this row's A-columns lies in it
+            const bool full = (W.c0 == 0 && W.c1 >= ncols);
+            int nocc = 0;
+            if (short_row) {
+                int32_t s = rk_bs, e = rk_be;
+                if (!full && a0 + l < a1) {
+                    s = have_cursor ? rk_cur : lower_bound_dev(bi, rk_bs, rk_be, W.c0);
+                    e = gallop_dev(bi, s, rk_be, W.c1);
+                    rk_cur = e;
                 }
-                heavy_batch<T>(L, bi, bv, s, e, a, W.c0, nocc, cnt);
+                heavy_batch<T>(L, bi, bv, s, e, rk_a, W.c0, nocc, cnt);
+            } else {
+                const bool last = (wi == I.wend - 1);
+                for (int32_t pb = a0; pb < a1; pb += kWave) {
+                    const int32_t p = pb + l;
+                    int32_t s = 0, e = 0;
+                    double a = 0.0;
+                    if (p < a1) {
+                        const int32_t k = ai[p];
+                        a = av[p];
+                        const int32_t bs = bp[k], be = bp[k + 1];
+                        if (full) {
+                            s = bs;
+                            e = be;
+                        } else {
+                            s = have_cursor ? cur[p - a0] : lower_bound_dev(bi, bs, be, W.c0);
+                            e = gallop_dev(bi, s, be, W.c1);
+                            if (!last) cur[p - a0] = e;
+                        }
+                    }
+                    heavy_batch<T>(L, bi, bv, s, e, a, W.c0, nocc, cnt);
+                }
             }
-        }
-        if (!full) have_cursor = true;
-        wave_sync();
-        // prefix popcounts of the presence words (8 words per lane)
-        const int nwords = (W.c1 - W.c0 + 63) >> 6;
-        int32_t wc[kBmWords / kWave];
-        int32_t tsum = 0;
-#pragma unroll
-        for (int u = 0; u < kBmWords / kWave; ++u) {
-            const int wd = l * (kBmWords / kWave) + u;
-            wc[u] = wd < nwords ? __popcll(L.bm[wd]) : 0;
-            tsum += wc[u];
-        }
-        const int32_t tinc = wave_incl_scan(tsum);
-        {
-            int32_t run = tinc - tsum;
+            if (!full) have_cursor = true;
+            wave_sync();
+            // prefix popcounts of the presence words (8 words per lane)
+            const int nwords = (W.c1 - W.c0 + 63) >> 6;
+            int32_t wc[kBmWords / kWave];
+            int32_t tsum = 0;
 #pragma unroll
             for (int u = 0; u < kBmWords / kWave; ++u) {
-                L.wpre[l * (kBmWords / kWave) + u] = run;
-                run += wc[u];
+                const int wd = l * (kBmWords / kWave) + u;
+                wc[u] = wd < nwords ? __popcll(L.bm[wd]) : 0;
+                tsum += wc[u];
             }
-        }
-        wave_sync();
-        if (l == 0 && nocc != W.cnt) atomicOr(&cnt->overflow, 4);
-        int32_t *oi = oi_row + W.off;
-        double *ov = ov_row + W.off;
-        const int nlist = nocc < T / 2 ? nocc : T / 2;
-        for (int t = l; t < nlist; t += kWave) {
-            const int32_t h = L.occ[t];
-            const int32_t k = L.keys[h];
-            const int32_t bpos = k - W.c0;
-            const int wd = bpos >> 6;
-            const int32_t r = L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (bpos & 63)) - 1ull));
-            if (r < W.cnt) {  // never write past the window
-                oi[r] = k;
-                ov[r] = L.vals[h];
+            const int32_t tinc = wave_incl_scan(tsum);
+            {
+                int32_t run = tinc - tsum;
+#pragma unroll
+                for (int u = 0; u < kBmWords / kWave; ++u) {
+                    L.wpre[l * (kBmWords / kWave) + u] = run;
+                    run += wc[u];
+                }
             }
+            wave_sync();
+            if (l == 0 && nocc != W.cnt) atomicOr(&cnt->overflow, 4);
+            int32_t *oi = oi_row + W.off;
+            double *ov = ov_row + W.off;
+            for (int h = l; h < T; h += kWave) {
+                const int32_t k = L.keys[h];
+                if (k == kEmpty) continue;
+                const int32_t bpos = k - W.c0;
+                const int wd = bpos >> 6;
+                const int32_t rr = L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (bpos & 63)) - 1ull));
+                if (rr < W.cnt) {  // never write past the window
+                    oi[rr] = k;
+                    ov[rr] = L.vals[h];
+                }
+            }
+            wave_sync();
+            for (int t = l; t < T; t += kWave) L.keys[t] = kEmpty;
+            for (int t = l; t < nwords; t += kWave) L.bm[t] = 0ull;
+            wave_sync();
         }
-        wave_sync();
-        for (int t = l; t < nlist; t += kWave) L.keys[L.occ[t]] = kEmpty;
-        for (int t = l; t < nwords; t += kWave) L.bm[t] = 0ull;
-        wave_sync();
     }
 }
 
-template <int T>
-void launch_sym_light(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, const int32_t *rows, int32_t n,
-                      int64_t *rownnz) {
-    TR_LAUNCH(ctx, "sym_light", k_sym_light<T>, div_up((uint64_t)n, 4), 256, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
-              B->idx_vec, rows, n, rownnz);
+}  // namespace
+
+// ======================================================================= host side
+
+void *Workspace::get(int slot, size_t bytes) {
+    if (!owned) return scratch(ctx, slot, bytes);
+    if (bytes == 0) bytes = 1;
+    if (sz[slot] < bytes) {
+        if (p[slot]) ctx->pool.release(p[slot]);
+        p[slot] = nullptr;
+        sz[slot] = 0;
+        p[slot] = ctx->pool.alloc(bytes);
+        sz[slot] = bytes;
+    }
+    return p[slot];
+}
+
+void Workspace::release() {
+    if (!owned || !ctx) return;
+    for (size_t s = 0; s < p.size(); ++s)
+        if (p[s]) {
+            try {
+                ctx->pool.release(p[s]);
+            } catch (...) {
+            }
+            p[s] = nullptr;
+            sz[s] = 0;
+        }
+}
+
+namespace {
+
+uint32_t cap_grid(int64_t need, int64_t cap) {
+    if (need <= 0) return 0;
+    return (uint32_t)std::min<int64_t>(need, cap);
 }
 
 template <int T>
-void launch_num_light(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, const int32_t *rows, int32_t n,
-                      const int64_t *rowptr, const Panel *panels, int np, Counters *cnt) {
-    TR_LAUNCH(ctx, "num_light", k_num_light<T>, div_up((uint64_t)n, 4), 256, 0, A->ptr_vec, A->idx_vec, A->val,
-              B->ptr_vec, B->idx_vec, B->val, rows, n, rowptr, panels, np, cnt);
+void launch_sym_light(tr_ctx *ctx, MulPlan &pl, int c) {
+    const int32_t n = pl.h.cls[c];
+    TR_LAUNCH(ctx, "sym_light", k_sym_light<T>, cap_grid(div_up((uint64_t)n, 4), 1 << 20), 256, 0, pl.d_tasks,
+              pl.nt, pl.lists + (int64_t)c * pl.rows_total, pl.cnt, c, pl.rownnz);
 }
 
-struct Plan {
-    int32_t m = 0, n = 0;
-    int64_t *flops = nullptr;
-    int32_t *lists = nullptr;
-    Counters *cnt = nullptr;
-    Counters h{};
-    int64_t *rownnz = nullptr;
-    int64_t *rowptr = nullptr;
-    Window *wins = nullptr;
-    Item *items = nullptr;
-    int32_t win_cap = 0, item_cap = 0;
-    int64_t nnz = 0;
-};
+template <int T>
+void launch_num_light(tr_ctx *ctx, MulPlan &pl, int c, Panel *panels, int np) {
+    const int32_t n = pl.h.cls[c];
+    TR_LAUNCH(ctx, "num_light", k_num_light<T>, cap_grid(div_up((uint64_t)n, 4), 1 << 20), 256, 0, pl.d_tasks,
+              pl.nt, pl.lists + (int64_t)c * pl.rows_total, pl.cnt, c, pl.rowptr, panels, np);
+}
+
+}  // namespace
+
+void mul_setup(tr_ctx *ctx, Workspace &ws, MulPlan &pl, const std::vector<TaskDev> &tasks) {
+    pl.nt = (int)tasks.size();
+    pl.h_tasks = tasks;
+    int64_t rows = 0;
+    for (auto &t : pl.h_tasks) {
+        t.row_base = (int32_t)rows;
+        rows += t.m;
+    }
+    TR_REQUIRE(rows < INT32_MAX, "spgemm: total rows of the batch exceed INT32_MAX");
+    pl.rows_total = (int32_t)rows;
+    const size_t R = (size_t)rows + 1;
+    pl.d_tasks = (TaskDev *)ws.get(S_PANELS + 0, sizeof(TaskDev) * (size_t)std::max(pl.nt, 1));
+    pl.flops = (int64_t *)ws.get(S_FLOPS, sizeof(int64_t) * R);
+    pl.lists = (int32_t *)ws.get(S_BINS, sizeof(int32_t) * (size_t)kNumClasses * R);
+    pl.cnt = (Counters *)ws.get(S_BINCNT, sizeof(Counters));
+    pl.rownnz = (int64_t *)ws.get(S_ROWNNZ, sizeof(int64_t) * R);
+    pl.rowptr = (int64_t *)ws.get(S_ROWPTR64, sizeof(int64_t) * R);
+    pl.scan_tmp = ws.owned ? (int64_t *)ws.get(S_SCAN_TMP, scan_tmp_bytes(rows)) : nullptr;
+    TR_HIP(hipMemcpyAsync(pl.d_tasks, pl.h_tasks.data(), sizeof(TaskDev) * (size_t)pl.nt, hipMemcpyHostToDevice,
+                          ctx->stream));
+}
+
+void mul_analyze(tr_ctx *ctx, MulPlan &pl) {
+    TR_LAUNCH(ctx, "zero_counters", k_zero_counters, 1, 64, 0, pl.cnt);
+    TR_LAUNCH(ctx, "zero_rownnz", k_zero_i64, cap_grid(div_up((uint64_t)pl.rows_total + 1, 256), 4096), 256, 0,
+              pl.rownnz, (int64_t)pl.rows_total + 1);
+    TR_LAUNCH(ctx, "row_flops", k_row_flops, div_up((uint64_t)pl.rows_total, kRowsPerBlock), kRowsPerBlock, 0,
+              pl.d_tasks, pl.nt, pl.rows_total, pl.flops, pl.lists, pl.cnt);
+}
+
+void mul_read_counts(tr_ctx *ctx, MulPlan &pl) { read_array(ctx, &pl.h, pl.cnt, sizeof(Counters)); }
+
+void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool capture) {
+    if (pl.h.cls[0]) launch_sym_light<64>(ctx, pl, 0);
+    if (pl.h.cls[1]) launch_sym_light<128>(ctx, pl, 1);
+    if (pl.h.cls[2]) launch_sym_light<256>(ctx, pl, 2);
+    if (pl.h.cls[3]) launch_sym_light<512>(ctx, pl, 3);
+    if (pl.h.cls[4]) launch_sym_light<1024>(ctx, pl, 4);
+    if (pl.h.cls[5]) launch_sym_light<2048>(ctx, pl, 5);
+    const int32_t nh = pl.h.cls[kHeavyClass];
+    if (nh) {
+        if (emit && !capture) {
+            int32_t max_n = 0;
+            for (auto &t : pl.h_tasks) max_n = std::max(max_n, t.n);
+            // windows per pass <= ceil(pass_nnz / H) + width splits
+            const int64_t passes = ((int64_t)max_n + (1 << kPassBits) - 1) >> kPassBits;
+            const int64_t wsplits = (int64_t)max_n / kWinMaxWidth + 1;
+            const int64_t wcap = (int64_t)(pl.h.heavy_flops / kWinH) + (int64_t)nh * (2 * passes + wsplits) + 16;
+            const int64_t icap = wcap / kItemWindows + (int64_t)nh * passes + 16;
+            TR_REQUIRE(wcap < INT32_MAX && icap < INT32_MAX, "spgemm: heavy work list exceeds int32 capacity");
+            pl.win_cap = (int32_t)wcap;
+            pl.item_cap = (int32_t)icap;
+            pl.wins = (Window *)ws.get(S_WINDOWS, sizeof(Window) * (size_t)wcap);
+            pl.items = (Item *)ws.get(S_ITEMS, sizeof(Item) * (size_t)icap);
+        }
+        TR_LAUNCH(ctx, "sym_heavy", k_sym_heavy, cap_grid(nh, (int64_t)ctx->cu_count * 2), kHeavyThreads, 0,
+                  pl.d_tasks, pl.nt, pl.lists + (int64_t)kHeavyClass * pl.rows_total, pl.cnt, pl.rownnz,
+                  emit ? 1 : 0, (int32_t)kWinH, pl.wins, pl.win_cap, pl.items, pl.item_cap);
+    }
+    launch_exclusive_scan_i64(ctx, pl.rownnz, pl.rowptr, pl.rows_total, pl.scan_tmp);
+}
+
+int64_t mul_cursor_prep(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool capture) {
+    if (!pl.h.cls[kHeavyClass]) return 0;
+    if (!capture) {
+        pl.clen = (int32_t *)ws.get(S_MISC, sizeof(int32_t) * ((size_t)pl.item_cap + 1));
+        pl.cbase = (int64_t *)ws.get(S_MISC2, sizeof(int64_t) * ((size_t)pl.item_cap + 1));
+    }
+    if (ws.owned && !capture) pl.scan_tmp = (int64_t *)ws.get(S_SCAN_TMP, scan_tmp_bytes(std::max<int64_t>(pl.item_cap, pl.rows_total)));
+    TR_LAUNCH(ctx, "item_cursor_len", k_item_cursor_len, div_up((uint64_t)pl.item_cap, 256), 256, 0, pl.items, pl.cnt,
+              pl.d_tasks, pl.nt, pl.item_cap, pl.clen);
+    launch_exclusive_scan_i32_to_i64(ctx, pl.clen, pl.cbase, pl.item_cap, pl.scan_tmp);
+    if (capture) return pl.cursor_cap;
+    return read_scalar<int64_t>(ctx, pl.cbase + pl.item_cap);
+}
+
+void mul_panel_bases(tr_ctx *ctx, MulPlan &pl, Panel *d_panels, int np) {
+    TR_LAUNCH(ctx, "panel_bases", k_panel_bases, div_up((uint64_t)np, 64), 64, 0, d_panels, np, pl.rowptr);
+}
+
+void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *dp, int np) {
+    if (pl.h.cls[0]) launch_num_light<64>(ctx, pl, 0, dp, np);
+    if (pl.h.cls[1]) launch_num_light<128>(ctx, pl, 1, dp, np);
+    if (pl.h.cls[2]) launch_num_light<256>(ctx, pl, 2, dp, np);
+    if (pl.h.cls[3]) launch_num_light<512>(ctx, pl, 3, dp, np);
+    if (pl.h.cls[4]) launch_num_light<1024>(ctx, pl, 4, dp, np);
+    if (pl.h.cls[5]) launch_num_light<2048>(ctx, pl, 5, dp, np);
+    if (pl.h.cls[kHeavyClass] && pl.h.items > 0) {
+        TR_LAUNCH(ctx, "num_heavy", k_num_heavy<kN2T>, cap_grid(pl.h.items, 1 << 24), kWave, 0, pl.d_tasks, pl.nt,
+                  pl.items, pl.cnt, pl.wins, pl.cbase, pl.cursors, pl.rowptr, dp, np);
+    }
+}
+
+void mul_check(tr_ctx *ctx, MulPlan &pl) {
+    const int32_t ov = read_scalar<int32_t>(ctx, &pl.cnt->overflow);
+    if (ov & 1) fail(TR_ERR_HIP, "spgemm: heavy work list capacity exceeded (internal bound violated)");
+    if (ov & 14) fail(TR_ERR_HIP, "spgemm: numeric row counts disagree with the symbolic phase (internal error)");
+}
+
+namespace {
+
+TaskDev task_of_pair(const tr_csx_t *A, const tr_csx_t *B) {
+    TaskDev t{};
+    t.ap = A->ptr_vec;
+    t.ai = A->idx_vec;
+    t.av = A->val;
+    t.bp = B->ptr_vec;
+    t.bi = B->idx_vec;
+    t.bv = B->val;
+    t.m = A->nrows;
+    t.n = B->ncols;
+    return t;
+}
+
+}  // namespace
 
 void validate_operands(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B) {
     check_csx(A, "A");
     check_csx(B, "B");
     TR_REQUIRE(A->majordim == TR_ROWS && B->majordim == TR_ROWS, "spgemm operands must be CSR (TR_ROWS)");
     TR_REQUIRE(A->ncols == B->nrows, "spgemm: A.ncols != B.nrows");
-    int32_t *st = (int32_t *)scratch(ctx, S_PANELS, 64);
+    int32_t *st = (int32_t *)scratch(ctx, S_TMP_CSX, 64);
     TR_HIP(hipMemsetAsync(st, 0, 2 * sizeof(int32_t), ctx->stream));
     launch_validate(ctx, A, st);
     launch_validate(ctx, B, st + 1);
@@ -807,82 +953,31 @@ void validate_operands(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B) {
     if (hs[0] & 4) fail(TR_ERR_INVALID, "spgemm: A has unsorted or duplicate columns; call tr_csx_sort_rows");
 }
 
-// row analysis + symbolic; leaves per-row nnz and 64-bit offsets in the plan
-void symbolic(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, Plan &pl, bool emit) {
-    pl.m = A->nrows;
-    pl.n = B->ncols;
-    const int32_t m = pl.m;
-    pl.flops = (int64_t *)scratch(ctx, S_FLOPS, sizeof(int64_t) * ((size_t)m + 1));
-    pl.lists = (int32_t *)scratch(ctx, S_BINS, sizeof(int32_t) * (size_t)kNumClasses * ((size_t)m + 1));
-    pl.cnt = (Counters *)scratch(ctx, S_BINCNT, sizeof(Counters));
-    pl.rownnz = (int64_t *)scratch(ctx, S_ROWNNZ, sizeof(int64_t) * ((size_t)m + 1));
-    pl.rowptr = (int64_t *)scratch(ctx, S_ROWPTR64, sizeof(int64_t) * ((size_t)m + 1));
-    TR_HIP(hipMemsetAsync(pl.cnt, 0, sizeof(Counters), ctx->stream));
-    TR_HIP(hipMemsetAsync(pl.rownnz, 0, sizeof(int64_t) * ((size_t)m + 1), ctx->stream));
-    if (m > 0 && A->nnz > 0 && B->nnz > 0) {
-        TR_LAUNCH(ctx, "row_flops", k_row_flops, div_up((uint64_t)m, kRowsPerBlock), kRowsPerBlock, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
-                  m, pl.n, pl.flops, pl.lists, pl.cnt);
-    }
-    read_array(ctx, &pl.h, pl.cnt, sizeof(Counters));
-    const int32_t *L = pl.lists;
-    const int64_t M = m;
-    if (pl.h.cls[0]) launch_sym_light<64>(ctx, A, B, L + 0 * M, pl.h.cls[0], pl.rownnz);
-    if (pl.h.cls[1]) launch_sym_light<128>(ctx, A, B, L + 1 * M, pl.h.cls[1], pl.rownnz);
-    if (pl.h.cls[2]) launch_sym_light<256>(ctx, A, B, L + 2 * M, pl.h.cls[2], pl.rownnz);
-    if (pl.h.cls[3]) launch_sym_light<512>(ctx, A, B, L + 3 * M, pl.h.cls[3], pl.rownnz);
-    if (pl.h.cls[4]) launch_sym_light<1024>(ctx, A, B, L + 4 * M, pl.h.cls[4], pl.rownnz);
-    if (pl.h.cls[5]) launch_sym_light<2048>(ctx, A, B, L + 5 * M, pl.h.cls[5], pl.rownnz);
-    const int32_t nh = pl.h.cls[kHeavyClass];
-    if (nh) {
-        if (emit) {
-            // windows per pass <= ceil(pass_nnz / H) + (width splits) <=
-            // pass_flops / H + 1 + pass_width / kWinMaxWidth
-            const int64_t passes = ((int64_t)pl.n + (1 << kPassBits) - 1) >> kPassBits;
-            const int64_t wsplits = (int64_t)pl.n / kWinMaxWidth + 1;
-            int64_t wcap = (int64_t)(pl.h.heavy_flops / kWinH) + (int64_t)nh * (2 * passes + wsplits) + 16;
-            int64_t icap = wcap / kItemWindows + (int64_t)nh * passes + 16;
-            TR_REQUIRE(wcap < INT32_MAX && icap < INT32_MAX, "spgemm: heavy work list exceeds int32 capacity");
-            pl.win_cap = (int32_t)wcap;
-            pl.item_cap = (int32_t)icap;
-            pl.wins = (Window *)scratch(ctx, S_WINDOWS, sizeof(Window) * (size_t)wcap);
-            pl.items = (Item *)scratch(ctx, S_ITEMS, sizeof(Item) * (size_t)icap);
-        }
-        TR_LAUNCH(ctx, "sym_heavy", k_sym_heavy, (uint32_t)nh, kHeavyThreads, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
-                  B->idx_vec, pl.n, L + (int64_t)kHeavyClass * M, nh, pl.rownnz, emit ? 1 : 0, (int32_t)kWinH,
-                  pl.wins, pl.win_cap,
-                  pl.items, pl.item_cap, pl.cnt);
-    }
-    launch_exclusive_scan_i64(ctx, pl.rownnz, pl.rowptr, m);
-}
-
-}  // namespace
-
 int64_t spgemm_flops(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B) {
     validate_operands(ctx, A, B);
-    Plan pl;
-    const int32_t m = A->nrows;
-    pl.flops = (int64_t *)scratch(ctx, S_FLOPS, sizeof(int64_t) * ((size_t)m + 1));
-    pl.lists = (int32_t *)scratch(ctx, S_BINS, sizeof(int32_t) * (size_t)kNumClasses * ((size_t)m + 1));
-    pl.cnt = (Counters *)scratch(ctx, S_BINCNT, sizeof(Counters));
-    TR_HIP(hipMemsetAsync(pl.cnt, 0, sizeof(Counters), ctx->stream));
-    if (m > 0 && A->nnz > 0 && B->nnz > 0)
-        TR_LAUNCH(ctx, "row_flops", k_row_flops, div_up((uint64_t)m, kRowsPerBlock), kRowsPerBlock, 0, A->ptr_vec, A->idx_vec, B->ptr_vec,
-                  m, B->ncols, pl.flops, pl.lists, pl.cnt);
-    read_array(ctx, &pl.h, pl.cnt, sizeof(Counters));
+    Workspace ws(ctx, false);
+    MulPlan pl;
+    mul_setup(ctx, ws, pl, {task_of_pair(A, B)});
+    mul_analyze(ctx, pl);
+    mul_read_counts(ctx, pl);
     return (int64_t)pl.h.flops;
 }
 
 void spgemm_symbolic_only(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *nnz, size_t *bytes) {
     validate_operands(ctx, A, B);
-    Plan pl;
-    symbolic(ctx, A, B, pl, false);
-    int64_t total = read_scalar<int64_t>(ctx, pl.rowptr + pl.m);
+    Workspace ws(ctx, false);
+    MulPlan pl;
+    mul_setup(ctx, ws, pl, {task_of_pair(A, B)});
+    mul_analyze(ctx, pl);
+    mul_read_counts(ctx, pl);
+    mul_symbolic(ctx, ws, pl, false, false);
+    const int64_t total = read_scalar<int64_t>(ctx, pl.rowptr + pl.rows_total);
     if (nnz) *nnz = total;
     if (bytes) {
         // output tile + work lists + cursors bound + row arrays
-        size_t out = (size_t)total * 12 + ((size_t)pl.m + 1) * 4;
-        size_t heavy = (size_t)(pl.h.heavy_flops / kWinH + 16) * (sizeof(Window) + sizeof(Item) / kItemWindows);
-        size_t rows = ((size_t)pl.m + 1) * (8 * 3 + 4 * kNumClasses);
+        const size_t out = (size_t)total * 12 + ((size_t)pl.rows_total + 1) * 4;
+        const size_t heavy = (size_t)(pl.h.heavy_flops / kWinH + 16) * (sizeof(Window) + sizeof(Item));
+        const size_t rows = ((size_t)pl.rows_total + 1) * (8 * 3 + 4 * kNumClasses);
         *bytes = out + heavy + rows + (size_t)A->nnz * 4;
     }
 }
@@ -892,43 +987,23 @@ int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max
     TR_REQUIRE(panels && row_begin && npanels && max_panels >= 1, "spgemm: null panel outputs");
     TR_REQUIRE(max_panel_nnz >= 1 && max_panel_nnz <= INT32_MAX, "spgemm: max_panel_nnz must be in [1, INT32_MAX]");
     validate_operands(ctx, A, B);
-    Plan pl;
-    symbolic(ctx, A, B, pl, true);
-    const int32_t m = pl.m;
-    const int32_t nh = pl.h.cls[kHeavyClass];
-
-    // cursor slots of multi-window items
-    int64_t *cbase = nullptr;
-    int32_t *clen = nullptr;
-    if (nh) {
-        clen = (int32_t *)scratch(ctx, S_MISC, sizeof(int32_t) * ((size_t)pl.item_cap + 1));
-        cbase = (int64_t *)scratch(ctx, S_MISC2, sizeof(int64_t) * ((size_t)pl.item_cap + 1));
-        TR_LAUNCH(ctx, "item_cursor_len", k_item_cursor_len, div_up((uint64_t)pl.item_cap, 256), 256, 0, pl.items,
-                  pl.cnt, A->ptr_vec, pl.item_cap, clen);
-        launch_exclusive_scan_i32_to_i64(ctx, clen, cbase, pl.item_cap);
-    }
-    // one read-back: totals
-    struct Tot {
-        int64_t nnz;
-        int64_t cursors;
-    } *ht = (Tot *)pinned(ctx, sizeof(Tot) + sizeof(Counters));
-    TR_HIP(hipMemcpyAsync(&ht->nnz, pl.rowptr + m, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    if (nh)
-        TR_HIP(hipMemcpyAsync(&ht->cursors, cbase + pl.item_cap, sizeof(int64_t), hipMemcpyDeviceToHost,
-                              ctx->stream));
-    Counters *hc = (Counters *)(ht + 1);
-    TR_HIP(hipMemcpyAsync(hc, pl.cnt, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
-    TR_HIP(hipStreamSynchronize(ctx->stream));
-    const int64_t total = ht->nnz;
-    const int64_t ncur = nh ? ht->cursors : 0;
-    pl.h = *hc;
+    Workspace ws(ctx, false);
+    MulPlan pl;
+    mul_setup(ctx, ws, pl, {task_of_pair(A, B)});
+    mul_analyze(ctx, pl);
+    mul_read_counts(ctx, pl);
+    mul_symbolic(ctx, ws, pl, true, false);
+    const int64_t ncur = mul_cursor_prep(ctx, ws, pl, false);
+    const int32_t m = pl.rows_total;
+    const int64_t total = read_scalar<int64_t>(ctx, pl.rowptr + m);
+    mul_read_counts(ctx, pl);
     if (pl.h.overflow & 1) fail(TR_ERR_HIP, "spgemm: heavy work list capacity exceeded (internal bound violated)");
 
     // panel split
-    std::vector<int32_t> rb;
-    rb.push_back(0);
+    std::vector<int32_t> rb{0};
+    std::vector<int64_t> hrp;
     if (total > max_panel_nnz) {
-        std::vector<int64_t> hrp((size_t)m + 1);
+        hrp.resize((size_t)m + 1);
         read_array(ctx, hrp.data(), pl.rowptr, sizeof(int64_t) * ((size_t)m + 1));
         int64_t start = 0;
         for (int32_t r = 0; r < m; ++r) {
@@ -947,23 +1022,14 @@ int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max
         fail(TR_ERR_OVERFLOW, "spgemm: nnz(C) = " + std::to_string(total) + " needs " + std::to_string(np) +
                                   " panels of at most " + std::to_string(max_panel_nnz) + " entries");
     }
-    std::vector<int64_t> nb(np + 1);
-    {
-        if (np == 1) {
-            nb[0] = 0;
-            nb[1] = total;
-        } else {
-            std::vector<int64_t> hrp((size_t)m + 1);
-            read_array(ctx, hrp.data(), pl.rowptr, sizeof(int64_t) * ((size_t)m + 1));
-            for (int32_t p = 0; p <= np; ++p) nb[p] = hrp[rb[p]];
-        }
-    }
     std::vector<Panel> hp(np);
     for (int32_t p = 0; p < np; ++p) {
+        const int64_t nb0 = np == 1 ? 0 : hrp[rb[p]];
+        const int64_t nb1 = np == 1 ? total : hrp[rb[p + 1]];
         tr_csx_t *C = &panels[p];
-        alloc_tile(ctx, TR_ROWS, rb[p + 1] - rb[p], pl.n, nb[p + 1] - nb[p], C);
+        alloc_tile(ctx, TR_ROWS, rb[p + 1] - rb[p], pl.h_tasks[0].n, nb1 - nb0, C);
         launch_rowptr_i64_to_i32(ctx, pl.rowptr, rb[p], rb[p + 1], C->ptr_vec);
-        hp[p].nnz_begin = nb[p];
+        hp[p].nnz_begin = nb0;
         hp[p].row_begin = rb[p];
         hp[p].pad = 0;
         hp[p].idx = C->idx_vec;
@@ -972,27 +1038,12 @@ int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max
     }
     row_begin[np] = m;
     *npanels = np;
-    Panel *dp = (Panel *)scratch(ctx, S_PANELS, sizeof(Panel) * (size_t)np);
+    Panel *dp = (Panel *)ws.get(S_TMP_CSX, sizeof(Panel) * (size_t)np);
     TR_HIP(hipMemcpyAsync(dp, hp.data(), sizeof(Panel) * (size_t)np, hipMemcpyHostToDevice, ctx->stream));
-
-    // numeric
-    const int32_t *L = pl.lists;
-    const int64_t M = m;
-    if (pl.h.cls[0]) launch_num_light<64>(ctx, A, B, L + 0 * M, pl.h.cls[0], pl.rowptr, dp, np, pl.cnt);
-    if (pl.h.cls[1]) launch_num_light<128>(ctx, A, B, L + 1 * M, pl.h.cls[1], pl.rowptr, dp, np, pl.cnt);
-    if (pl.h.cls[2]) launch_num_light<256>(ctx, A, B, L + 2 * M, pl.h.cls[2], pl.rowptr, dp, np, pl.cnt);
-    if (pl.h.cls[3]) launch_num_light<512>(ctx, A, B, L + 3 * M, pl.h.cls[3], pl.rowptr, dp, np, pl.cnt);
-    if (pl.h.cls[4]) launch_num_light<1024>(ctx, A, B, L + 4 * M, pl.h.cls[4], pl.rowptr, dp, np, pl.cnt);
-    if (pl.h.cls[5]) launch_num_light<2048>(ctx, A, B, L + 5 * M, pl.h.cls[5], pl.rowptr, dp, np, pl.cnt);
-    if (nh && pl.h.items > 0) {
-        int32_t *cursors = (int32_t *)scratch(ctx, S_CURSORS, sizeof(int32_t) * (size_t)std::max<int64_t>(ncur, 1));
-        TR_LAUNCH(ctx, "num_heavy", k_num_heavy<kN2T>, (uint32_t)pl.h.items, kWave, 0, A->ptr_vec, A->idx_vec,
-                  A->val, B->ptr_vec, B->idx_vec, B->val, pl.n, pl.items, pl.wins, cbase, cursors, pl.rowptr, dp, np,
-                  pl.cnt);
-    }
-    // internal consistency flags (symbolic vs numeric counts)
-    int32_t ov = read_scalar<int32_t>(ctx, &pl.cnt->overflow);
-    if (ov & 6) fail(TR_ERR_HIP, "spgemm: numeric row counts disagree with the symbolic phase (internal error)");
+    if (pl.h.cls[kHeavyClass] && pl.h.items > 0)
+        pl.cursors = (int32_t *)ws.get(S_CURSORS, sizeof(int32_t) * (size_t)std::max<int64_t>(ncur, 1));
+    mul_numeric(ctx, pl, dp, np);
+    mul_check(ctx, pl);
     return TR_OK;
 }
 

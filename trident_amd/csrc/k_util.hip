@@ -78,15 +78,17 @@ __global__ void k_scan_apply(const Tin *__restrict__ in, int64_t n, const int64_
     }
 }
 
+__global__ void k_store0_i64(int64_t *p) { *p = 0; }
+
 template <typename Tin>
-void scan_impl(tr_ctx *ctx, const Tin *in, int64_t *out, int64_t n) {
+void scan_impl(tr_ctx *ctx, const Tin *in, int64_t *out, int64_t n, int64_t *tmp) {
     // out[0..n] = exclusive prefix, out[n] = total
     int64_t ntiles = (n + kScanTile - 1) / kScanTile;
     if (ntiles == 0) {
-        TR_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), ctx->stream));
+        TR_LAUNCH(ctx, "scan_empty", k_store0_i64, 1, 1, 0, out);
         return;
     }
-    int64_t *sums = (int64_t *)scratch(ctx, S_SCAN_TMP, sizeof(int64_t) * (size_t)ntiles);
+    int64_t *sums = tmp ? tmp : (int64_t *)scratch(ctx, S_SCAN_TMP, sizeof(int64_t) * (size_t)ntiles);
     TR_LAUNCH(ctx, "scan_reduce", k_scan_reduce<Tin>, (uint32_t)ntiles, kScanThreads, 0, in, n, sums);
     TR_LAUNCH(ctx, "scan_tiles", k_scan_tiles, 1, kScanThreads, 0, sums, ntiles, out + n);
     TR_LAUNCH(ctx, "scan_apply", k_scan_apply<Tin>, (uint32_t)ntiles, kScanThreads, 0, in, n, sums, out);
@@ -271,11 +273,12 @@ __global__ void k_sort_classify(const int32_t *__restrict__ ptr, int32_t m, int3
 
 }  // namespace
 
-void launch_exclusive_scan_i64(tr_ctx *ctx, const int64_t *in, int64_t *out, int64_t n) {
-    scan_impl<int64_t>(ctx, in, out, n);
+size_t scan_tmp_bytes(int64_t n) { return sizeof(int64_t) * (size_t)((n + kScanTile - 1) / kScanTile + 1); }
+void launch_exclusive_scan_i64(tr_ctx *ctx, const int64_t *in, int64_t *out, int64_t n, int64_t *tmp) {
+    scan_impl<int64_t>(ctx, in, out, n, tmp);
 }
-void launch_exclusive_scan_i32_to_i64(tr_ctx *ctx, const int32_t *in, int64_t *out, int64_t n) {
-    scan_impl<int32_t>(ctx, in, out, n);
+void launch_exclusive_scan_i32_to_i64(tr_ctx *ctx, const int32_t *in, int64_t *out, int64_t n, int64_t *tmp) {
+    scan_impl<int32_t>(ctx, in, out, n, tmp);
 }
 
 void launch_fill_i32(tr_ctx *ctx, int32_t *p, int32_t v, int64_t n) {

@@ -1,0 +1,128 @@
+// Types shared by the multiply pipeline (k_spgemm.hip) and the batched
+// stage executor (k_batch.hip).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#ifndef TR_HEAVY_T
+#define TR_HEAVY_T 1024
+#endif
+#ifndef TR_WIN_H
+#define TR_WIN_H 384
+#endif
+
+namespace tr {
+
+// tuning constants (see DESIGN notes in the repository documentation)
+constexpr int kLightMaxF = 1024;   // symbolic light/heavy cut on row flops
+constexpr int kNumLight = 6;       // light classes: T = 64 << c, c < kNumLight
+constexpr int kHeavyClass = kNumLight;
+constexpr int kNumClasses = kNumLight + 1;
+constexpr int kPassBits = 20;      // heavy bitmap covers 2^20 columns per pass
+constexpr int kFineShift = 7;      // fine block = 128 columns
+constexpr int kWinH = TR_WIN_H;     // window packing quantum (entries)
+constexpr int kN2T = TR_HEAVY_T;   // heavy window hash slots (>= 2 * (kWinH + 127))
+static_assert(2 * (kWinH + 127) <= kN2T, "heavy table must hold a window at load <= 1/2");
+constexpr int kWinMaxWidth = 32768;                       // window width cap (columns)
+constexpr int kWidthShift = 15 - kFineShift;               // fine blocks per width quantum: 2^8
+constexpr int kBmWords = kWinMaxWidth / 64;                // presence words per window
+constexpr int kItemWindows = 16;   // windows per heavy work item
+constexpr int kHeavyThreads = 1024;
+
+struct Window {
+    int32_t c0, c1;   // column range [c0, c1)
+    int32_t off;      // offset of the window's first entry within the row
+    int32_t cnt;      // entries in the window
+};
+
+struct Item {
+    int32_t row;
+    int32_t wbeg, wend;  // window index range
+    int32_t pad;
+};
+
+struct Panel {
+    int64_t nnz_begin;   // global offset of the panel's first entry
+    int32_t row_begin;
+    int32_t pad;
+    int32_t *idx;
+    double *val;
+};
+
+// One product C_t = A_t * B_t of a batch (a plain call is a batch of one).
+// Rows of all tasks form one global row space: task t owns global rows
+// [row_base, row_base + m).
+struct TaskDev {
+    const int32_t *ap, *ai;
+    const double *av;
+    const int32_t *bp, *bi;
+    const double *bv;
+    int32_t m;         // rows of A_t
+    int32_t n;         // columns of B_t
+    int32_t row_base;  // first global row
+    int32_t pad;
+};
+
+// device counters written by the symbolic phase
+struct Counters {
+    int32_t cls[kNumClasses];        // rows per class
+    int32_t pad0;
+    unsigned long long flops;        // total multiplications
+    unsigned long long heavy_flops;  // multiplications of heavy rows
+    int32_t windows;                 // windows appended
+    int32_t items;                   // items appended
+    int32_t overflow;                // capacity exceeded
+    int32_t pad1;
+};
+
+
+// Buffers of one pipeline: the context's scratch slots for plain calls, or
+// owned allocations for batches (a captured graph needs fixed addresses).
+struct Workspace {
+    tr_ctx *ctx = nullptr;
+    bool owned = false;
+    std::vector<void *> p;
+    std::vector<size_t> sz;
+    Workspace(tr_ctx *c, bool own) : ctx(c), owned(own), p(S_NUM_SLOTS, nullptr), sz(S_NUM_SLOTS, 0) {}
+    void *get(int slot, size_t bytes);
+    void release();
+    ~Workspace() { release(); }
+    Workspace(const Workspace &) = delete;
+    Workspace &operator=(const Workspace &) = delete;
+};
+
+// One multiply job over a table of tasks.
+struct MulPlan {
+    int nt = 0;
+    int32_t rows_total = 0;
+    std::vector<TaskDev> h_tasks;
+    TaskDev *d_tasks = nullptr;
+    int64_t *flops = nullptr, *rownnz = nullptr, *rowptr = nullptr;
+    int32_t *lists = nullptr;
+    Counters *cnt = nullptr;
+    Counters h{};
+    Window *wins = nullptr;
+    Item *items = nullptr;
+    int32_t win_cap = 0, item_cap = 0;
+    int32_t *clen = nullptr;
+    int64_t *cbase = nullptr;
+    int32_t *cursors = nullptr;
+    int64_t cursor_cap = 0;
+    int64_t *scan_tmp = nullptr;  // owned scan temp (graph mode)
+    size_t scan_tmp_bytes = 0;
+};
+
+// Pipeline steps.  In eager mode the host reads counts between steps to size
+// grids and lists; in capture mode every size comes from a previous eager run
+// on the same operands.
+void mul_setup(tr_ctx *ctx, Workspace &ws, MulPlan &pl, const std::vector<TaskDev> &tasks);
+void mul_analyze(tr_ctx *ctx, MulPlan &pl);
+void mul_read_counts(tr_ctx *ctx, MulPlan &pl);
+void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool capture);
+int64_t mul_cursor_prep(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool capture);  // returns cursor total (eager)
+void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *d_panels, int np);
+void mul_panel_bases(tr_ctx *ctx, MulPlan &pl, Panel *d_panels, int np);
+void mul_check(tr_ctx *ctx, MulPlan &pl);  // reads the consistency flags (eager)
+
+}  // namespace tr

@@ -153,8 +153,12 @@ void check_csx(const tr_csx_t *m, const char *what);
 // ---- launchers implemented in the kernel translation units ----
 
 // k_util.hip
-void launch_exclusive_scan_i64(tr_ctx *ctx, const int64_t *in, int64_t *out, int64_t n);  // out has n+1
-void launch_exclusive_scan_i32_to_i64(tr_ctx *ctx, const int32_t *in, int64_t *out, int64_t n);
+// out[0..n] = exclusive prefix (out[n] = total).  tmp: caller-owned temp of
+// scan_tmp_bytes(n) bytes, or nullptr to use the context scratch.
+size_t scan_tmp_bytes(int64_t n);
+void launch_exclusive_scan_i64(tr_ctx *ctx, const int64_t *in, int64_t *out, int64_t n, int64_t *tmp = nullptr);
+void launch_exclusive_scan_i32_to_i64(tr_ctx *ctx, const int32_t *in, int64_t *out, int64_t n,
+                                      int64_t *tmp = nullptr);
 void launch_validate(tr_ctx *ctx, const tr_csx_t *m, int32_t *d_status);
 void launch_fill_i32(tr_ctx *ctx, int32_t *p, int32_t v, int64_t n);
 void launch_transpose(tr_ctx *ctx, const tr_csx_t *m, tr_csx_t *out);
@@ -167,7 +171,7 @@ void launch_bitmask_and(tr_ctx *ctx, const uint64_t *a, const uint64_t *b, int32
 int64_t compress(tr_ctx *ctx, const tr_csx_t *m, const uint64_t *mask, tr_csx_t *out);
 
 // k_spgemm.hip
-struct SpgemmPlan;
+void validate_operands(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B);
 int64_t spgemm_flops(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B);
 // Runs the symbolic phase and returns nnz(C) (and bytes needed).
 void spgemm_symbolic_only(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *nnz, size_t *bytes);
