@@ -124,6 +124,33 @@ int tr_symbolic(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *nnz
 /* Number of multiplications sum_k nnz(A[:,k]) * nnz(B[k,:]) (device pass). */
 int tr_spgemm_flops(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *flops);
 
+/* ---- batched stage executor ------------------------------------------------ */
+
+/* One SUMMA / Trident stage of tile products on this GPU, run as a single
+ * hipGraph launch: for every task t, C_out_t = C_in_t + compress(A_t) *
+ * compress(B_t).  This is the per-stage work each rank does through
+ * sp_mma_hybrid (src/hns_spgemm.cu:953-969, src/sparse_summa.cu:97-99), all
+ * tile products of the stage at once.
+ *  - A[t], B[t]: CSR tiles (borrowed, sorted), A[t].ncols == B[t].nrows;
+ *  - C_in: NULL or ntasks accumulators (borrowed; ptr_vec == NULL entries
+ *    are empty);
+ *  - compress_mode 0: no compression; 1: bitmask compression with
+ *    mask = non-empty columns of A[t] AND non-empty rows of B[t] (the filter
+ *    of SpaComm, include/spacomm.cuh:165-352, :962-1091).
+ * tr_batch_create plans the stage (one eager pass that sizes every buffer
+ * and already produces the results) and captures it; tr_batch_run replays
+ * the graph on the context stream with no host round trip.  Inputs must keep
+ * their shapes and addresses between create and run.  Results are
+ * batch-owned (tr_batch_result) and stay valid until the next run or
+ * tr_batch_destroy. */
+typedef struct tr_batch *tr_batch_t;
+int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_csx_t *B, const tr_csx_t *C_in,
+                    int32_t compress_mode, tr_batch_t *batch);
+int tr_batch_run(tr_batch_t batch);
+int tr_batch_result(tr_batch_t batch, int32_t task, tr_csx_t *C_out);
+int tr_batch_stats(tr_batch_t batch, int64_t *flops, int64_t *nnz_products, int64_t *nnz_out);
+int tr_batch_destroy(tr_batch_t batch);
+
 /* ---- bitmask tile compression (SpaComm) --------------------------------- */
 
 /* mask[w] bit b set iff segment 64*w+b of ptr is non-empty; ceil(n/64)

@@ -65,6 +65,11 @@ SIGNATURES = [
     ("tr_spadd", ctypes.c_int, [_vp, _csx, _csx]),
     ("tr_symbolic", ctypes.c_int, [_vp, _csx, _csx, _P(_i64), _P(_sz)]),
     ("tr_spgemm_flops", ctypes.c_int, [_vp, _csx, _csx, _P(_i64)]),
+    ("tr_batch_create", ctypes.c_int, [_vp, _i32, _csx, _csx, _csx, _i32, _P(_vp)]),
+    ("tr_batch_run", ctypes.c_int, [_vp]),
+    ("tr_batch_result", ctypes.c_int, [_vp, _i32, _csx]),
+    ("tr_batch_stats", ctypes.c_int, [_vp, _P(_i64), _P(_i64), _P(_i64)]),
+    ("tr_batch_destroy", ctypes.c_int, [_vp]),
     ("tr_bitmask_gen", ctypes.c_int, [_vp, _vp, _i32, _vp]),
     ("tr_bitmask_and", ctypes.c_int, [_vp, _vp, _vp, _i32, _vp]),
     ("tr_compress", ctypes.c_int, [_vp, _csx, _vp, _csx]),

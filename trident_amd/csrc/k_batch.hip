@@ -1,0 +1,488 @@
+// Batched stage executor: one SUMMA / Trident stage's tile products on one
+// GPU as a single hipGraph launch.
+//
+// For every task t of the stage (a tile product that the reference runs as
+// one sp_mma_hybrid call per stage per rank, src/hns_spgemm.cu:953-969,
+// src/sparse_summa.cu:97-99):
+//   1. bitmask compression (optional): mask_t = non-empty columns of A_t AND
+//      non-empty rows of B_t (the filter SpaComm derives, include/spacomm.cuh
+//      :165-352 and :962-1091); A_t keeps the entries of masked columns, B_t
+//      the masked rows;
+//   2. the product P_t = A_t * B_t through the multiply pipeline, all tasks
+//      in one task table, so every kernel launch covers all tile products
+//      (uneven products fill the GPU together instead of one after another);
+//   3. the accumulation C_t = C_in_t + P_t through the merge-path add, again
+//      one launch per phase over all tasks.
+// The plan step runs the whole stage once eagerly (host reads size every
+// list and output), then captures the same launch sequence, with every size
+// fixed, into a hipGraph; tr_batch_run replays it with no host round trip.
+// Inputs are borrowed and must not change shape between plan and run.
+#include "dev_common.h"
+#include "tr_internal.h"
+#include "spgemm_types.h"
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+namespace tr {
+namespace {
+
+using namespace dev;
+
+__global__ void k_fill_u64(uint64_t *__restrict__ p, int64_t n, uint64_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_col_presence(const int32_t *__restrict__ idx, int64_t nnz, uint64_t *__restrict__ mask) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = idx[p];
+        atomicOr((unsigned long long *)&mask[c >> 6], 1ull << (c & 63));
+    }
+}
+
+// mask[w] &= (bit b set iff row 64 w + b of ptr is non-empty)
+__global__ void k_and_row_presence(const int32_t *__restrict__ ptr, int32_t n, uint64_t *__restrict__ mask) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool nonempty = false;
+    if (i < n) nonempty = ptr[i] != ptr[i + 1];
+    const uint64_t word = ballot(nonempty);
+    if (lane_id() == 0 && i < n) mask[i >> 6] &= word;
+}
+
+// CSR column filter: keep the entries whose column bit is set.  A thread per
+// row for rows up to 32 entries, the whole wave for longer rows.
+template <bool kWrite>
+__global__ __launch_bounds__(256) void k_colfilter(const int32_t *__restrict__ ap, const int32_t *__restrict__ ai,
+                                                  const double *__restrict__ av, int32_t m,
+                                                  const uint64_t *__restrict__ mask, int32_t *__restrict__ kept,
+                                                  const int64_t *__restrict__ np64, int32_t *__restrict__ optr,
+                                                  int32_t *__restrict__ oi, double *__restrict__ ov) {
+    const int l = lane_id();
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t a0 = 0, a1 = 0;
+    if (r < m) {
+        a0 = ap[r];
+        a1 = ap[r + 1];
+    }
+    int64_t base = 0;
+    if (kWrite && r < m) {
+        base = np64[r];
+        optr[r] = (int32_t)base;
+        if (r == m - 1) optr[m] = (int32_t)np64[m];
+    }
+    int32_t c = 0;
+    if (a1 - a0 <= 32) {
+        for (int32_t p = a0; p < a1; ++p) {
+            const int32_t col = ai[p];
+            if ((mask[col >> 6] >> (col & 63)) & 1ull) {
+                if (kWrite) {
+                    oi[base + c] = col;
+                    ov[base + c] = av[p];
+                }
+                ++c;
+            }
+        }
+    }
+    uint64_t longm = ballot(a1 - a0 > 32);
+    while (longm) {
+        const int src = __ffsll((unsigned long long)longm) - 1;
+        longm &= longm - 1;
+        const int32_t s0 = __shfl(a0, src, kWave), s1 = __shfl(a1, src, kWave);
+        const int64_t sb = __shfl(base, src, kWave);
+        int32_t run = 0;
+        for (int32_t pb = s0; pb < s1; pb += kWave) {
+            const int32_t p = pb + l;
+            bool keep = false;
+            int32_t col = 0;
+            if (p < s1) {
+                col = ai[p];
+                keep = (mask[col >> 6] >> (col & 63)) & 1ull;
+            }
+            const uint64_t mk = ballot(keep);
+            if (kWrite && keep) {
+                const int64_t q = sb + run + rank_below(mk);
+                oi[q] = col;
+                ov[q] = av[p];
+            }
+            run += __popcll(mk);
+        }
+        if (l == src) c = run;
+    }
+    if (!kWrite && r < m) kept[r] = c;
+}
+
+__global__ void k_zero_i32(int32_t *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+}  // namespace
+}  // namespace tr
+
+using namespace tr;
+
+// Operands of one task after compression (device arrays, batch-owned when
+// compressed, borrowed otherwise).
+struct BatchOperand {
+    const int32_t *ptr = nullptr, *idx = nullptr;
+    const double *val = nullptr;
+    int32_t *optr = nullptr, *oidx = nullptr;
+    double *oval = nullptr;
+};
+
+struct tr_batch {
+    tr_ctx *ctx = nullptr;
+    int nt = 0;
+    int32_t mode = 0;
+    std::vector<tr_csx_t> A, B, Cin;
+    // compression
+    std::vector<uint64_t *> masks;
+    std::vector<BatchOperand> Ac, Bc;
+    std::vector<int32_t *> kept_tmp;
+    std::vector<int64_t *> np64;
+    std::vector<int64_t *> scan_tmp;
+    std::vector<void *> owned;  // pool blocks owned by the batch
+    int32_t *zero_ptr = nullptr;
+    // pipelines
+    std::unique_ptr<Workspace> ws_mul, ws_add;
+    MulPlan mp;
+    AddPlan ap;
+    Panel *d_panel = nullptr;
+    int64_t ncur = 0;
+    // product and output arenas
+    int32_t *p_idx = nullptr;
+    double *p_val = nullptr;
+    int64_t p_total = 0;
+    int32_t *o_idx = nullptr;
+    double *o_val = nullptr;
+    int32_t *o_ptr = nullptr;
+    int64_t o_total = 0;
+    std::vector<int64_t> o_base;     // per task: first entry in the output arena
+    std::vector<int64_t> optr_off;   // per task: first row pointer in o_ptr
+    std::vector<int64_t> o_nnz;
+    int64_t flops = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool ran = false;
+
+    void *own(size_t bytes) {
+        void *p = ctx->pool.alloc(bytes);
+        owned.push_back(p);
+        return p;
+    }
+    ~tr_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        ws_mul.reset();
+        ws_add.reset();
+        for (void *p : owned) {
+            try {
+                ctx->pool.release(p);
+            } catch (...) {
+            }
+        }
+    }
+};
+
+namespace {
+
+// enqueue compression for every task
+void batch_compress(tr_batch *b) {
+    tr_ctx *ctx = b->ctx;
+    for (int t = 0; t < b->nt; ++t) {
+        const tr_csx_t &A = b->A[t], &B = b->B[t];
+        const int32_t k = A.ncols;
+        const int64_t words = ((int64_t)k + 63) / 64;
+        uint64_t *mask = b->masks[t];
+        TR_LAUNCH(ctx, "mask_fill", k_fill_u64, div_up((uint64_t)std::max<int64_t>(words, 1), 256), 256, 0, mask,
+                  words, 0ull);
+        if (A.nnz)
+            TR_LAUNCH(ctx, "mask_cols", k_col_presence, div_up((uint64_t)std::min<int64_t>(A.nnz, 1 << 22), 256), 256,
+                      0, A.idx_vec, A.nnz, mask);
+        TR_LAUNCH(ctx, "mask_rows", k_and_row_presence, div_up((uint64_t)std::max(k, 1), 256), 256, 0, B.ptr_vec, k,
+                  mask);
+        // A: column filter
+        BatchOperand &ac = b->Ac[t];
+        TR_LAUNCH(ctx, "colfilter_count", k_colfilter<false>, div_up((uint64_t)std::max(A.nrows, 1), 256), 256, 0,
+                  A.ptr_vec, A.idx_vec, A.val, A.nrows, mask, b->kept_tmp[t], (const int64_t *)nullptr,
+                  (int32_t *)nullptr, (int32_t *)nullptr, (double *)nullptr);
+        launch_exclusive_scan_i32_to_i64(ctx, b->kept_tmp[t], b->np64[t], A.nrows, b->scan_tmp[t]);
+        TR_LAUNCH(ctx, "colfilter_write", k_colfilter<true>, div_up((uint64_t)std::max(A.nrows, 1), 256), 256, 0,
+                  A.ptr_vec, A.idx_vec, A.val, A.nrows, mask, (int32_t *)nullptr, b->np64[t], ac.optr, ac.oidx,
+                  ac.oval);
+        if (A.nrows == 0) TR_LAUNCH(ctx, "zero_ptr", k_zero_i32, 1, 64, 0, ac.optr, (int64_t)1);
+        // B: row segments
+        BatchOperand &bc = b->Bc[t];
+        if (B.nrows == 0) TR_LAUNCH(ctx, "zero_ptr", k_zero_i32, 1, 64, 0, bc.optr, (int64_t)1);
+        launch_compress_fixed(ctx, B.nrows, B.ptr_vec, B.idx_vec, B.val, mask, b->kept_tmp[t], b->np64[t],
+                              b->scan_tmp[t], bc.optr, bc.oidx, bc.oval);
+    }
+}
+
+// enqueue multiply + add; capture == true: sizes from the eager run
+void batch_body(tr_batch *b, bool capture) {
+    tr_ctx *ctx = b->ctx;
+    if (b->mode) batch_compress(b);
+    mul_analyze(ctx, b->mp);
+    if (!capture) mul_read_counts(ctx, b->mp);
+    mul_symbolic(ctx, *b->ws_mul, b->mp, true, capture);
+    const int64_t ncur = mul_cursor_prep(ctx, *b->ws_mul, b->mp, capture);
+    if (!capture) {
+        b->p_total = read_scalar<int64_t>(ctx, b->mp.rowptr + b->mp.rows_total);
+        mul_read_counts(ctx, b->mp);
+        if (b->mp.h.overflow & 1) fail(TR_ERR_HIP, "batch: heavy work list capacity exceeded");
+        b->mp.cursor_cap = ncur;
+        if (b->mp.h.cls[kHeavyClass] && b->mp.h.items > 0)
+            b->mp.cursors = (int32_t *)b->ws_mul->get(S_CURSORS, sizeof(int32_t) * (size_t)std::max<int64_t>(ncur, 1));
+        b->p_idx = (int32_t *)b->own(sizeof(int32_t) * (size_t)std::max<int64_t>(b->p_total, 1));
+        b->p_val = (double *)b->own(sizeof(double) * (size_t)std::max<int64_t>(b->p_total, 1));
+        Panel hp{};
+        hp.nnz_begin = 0;
+        hp.row_begin = 0;
+        hp.idx = b->p_idx;
+        hp.val = b->p_val;
+        b->d_panel = (Panel *)b->own(sizeof(Panel));
+        TR_HIP(hipMemcpyAsync(b->d_panel, &hp, sizeof(Panel), hipMemcpyHostToDevice, ctx->stream));
+        // per-task product sizes must fit int32 tiles
+        std::vector<int64_t> rp(b->nt + 1);
+        for (int t = 0; t <= b->nt; ++t) {
+            const int32_t g = t < b->nt ? b->mp.h_tasks[t].row_base : b->mp.rows_total;
+            rp[t] = read_scalar<int64_t>(ctx, b->mp.rowptr + g);
+        }
+        for (int t = 0; t < b->nt; ++t)
+            if (rp[t + 1] - rp[t] > INT32_MAX) fail(TR_ERR_OVERFLOW, "batch: a tile product exceeds INT32_MAX entries");
+    }
+    mul_numeric(ctx, b->mp, b->d_panel, 1);
+    if (!capture) {
+        mul_check(ctx, b->mp);
+        // add tasks: X = products (64-bit global rows), Y = C_in
+        std::vector<AddTaskDev> at(b->nt);
+        int64_t ny = 0;
+        b->optr_off.assign(b->nt, 0);
+        int64_t optr_total = 0;
+        for (int t = 0; t < b->nt; ++t) {
+            b->optr_off[t] = optr_total;
+            optr_total += b->A[t].nrows + 1;
+        }
+        b->o_ptr = (int32_t *)b->own(sizeof(int32_t) * (size_t)std::max<int64_t>(optr_total, 1));
+        for (int t = 0; t < b->nt; ++t) {
+            AddTaskDev &a = at[t];
+            a = AddTaskDev{};
+            a.xp64 = b->mp.rowptr;
+            a.xi = b->p_idx;
+            a.xv = b->p_val;
+            const tr_csx_t &C = b->Cin[t];
+            if (C.ptr_vec) {
+                a.yp = C.ptr_vec;
+                a.yi = C.idx_vec;
+                a.yv = C.val;
+                ny += C.nnz;
+            } else {
+                a.yp = b->zero_ptr;
+                a.yi = nullptr;
+                a.yv = nullptr;
+            }
+            a.optr = b->o_ptr + b->optr_off[t];
+            a.m = b->A[t].nrows;
+        }
+        add_setup(ctx, *b->ws_add, b->ap, at, b->p_total, ny);
+    }
+    add_count(ctx, b->ap, capture);
+    if (!capture) {
+        b->o_total = add_total(ctx, b->ap);
+        b->o_idx = (int32_t *)b->own(sizeof(int32_t) * (size_t)std::max<int64_t>(b->o_total, 1));
+        b->o_val = (double *)b->own(sizeof(double) * (size_t)std::max<int64_t>(b->o_total, 1));
+    }
+    add_write(ctx, b->ap, b->o_idx, b->o_val);
+}
+
+}  // namespace
+
+template <typename F>
+static int bguard(tr_ctx *ctx, F &&body) {
+    try {
+        body();
+        if (ctx) ctx->last_error.clear();
+        return TR_OK;
+    } catch (const Error &e) {
+        if (ctx) ctx->last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return TR_ERR_OOM;
+    } catch (...) {
+        if (ctx) ctx->last_error = "unexpected internal error";
+        return TR_ERR_HIP;
+    }
+}
+
+extern "C" {
+
+int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_csx_t *B, const tr_csx_t *C_in,
+                    int32_t compress_mode, tr_batch_t *out) {
+    if (!ctx || !out) return TR_ERR_INVALID;
+    *out = nullptr;
+    tr_batch *b = nullptr;
+    const int rc = bguard(ctx, [&] {
+        TR_REQUIRE(ntasks >= 1 && A && B, "tr_batch_create: need at least one task");
+        TR_REQUIRE(compress_mode == 0 || compress_mode == 1, "tr_batch_create: compress_mode must be 0 or 1");
+        b = new tr_batch();
+        b->ctx = ctx;
+        b->nt = ntasks;
+        b->mode = compress_mode;
+        b->A.assign(A, A + ntasks);
+        b->B.assign(B, B + ntasks);
+        b->Cin.resize(ntasks);
+        int32_t max_m = 0;
+        for (int t = 0; t < ntasks; ++t) {
+            b->Cin[t] = C_in ? C_in[t] : tr_csx_t{};
+            validate_operands(ctx, &b->A[t], &b->B[t]);
+            const tr_csx_t &C = b->Cin[t];
+            if (C.ptr_vec) {
+                check_csx(&C, "C_in");
+                TR_REQUIRE(C.majordim == TR_ROWS && C.nrows == b->A[t].nrows && C.ncols == b->B[t].ncols,
+                           "tr_batch_create: accumulator shape mismatch");
+                int32_t *st = (int32_t *)scratch(ctx, S_TMP_CSX, 64);
+                TR_HIP(hipMemsetAsync(st, 0, sizeof(int32_t), ctx->stream));
+                launch_validate(ctx, &C, st);
+                int32_t hs = 0;
+                read_array(ctx, &hs, st, sizeof(hs));
+                TR_REQUIRE(hs == 0, "tr_batch_create: accumulator is not a sorted CSR");
+            }
+            max_m = std::max(max_m, b->A[t].nrows);
+        }
+        b->zero_ptr = (int32_t *)b->own(sizeof(int32_t) * ((size_t)max_m + 1));
+        TR_HIP(hipMemsetAsync(b->zero_ptr, 0, sizeof(int32_t) * ((size_t)max_m + 1), ctx->stream));
+        // compression buffers (capacity = input nnz)
+        std::vector<TaskDev> tasks(ntasks);
+        b->masks.assign(ntasks, nullptr);
+        b->Ac.resize(ntasks);
+        b->Bc.resize(ntasks);
+        b->kept_tmp.assign(ntasks, nullptr);
+        b->np64.assign(ntasks, nullptr);
+        b->scan_tmp.assign(ntasks, nullptr);
+        for (int t = 0; t < ntasks; ++t) {
+            const tr_csx_t &a = b->A[t], &bb = b->B[t];
+            TaskDev &T = tasks[t];
+            T = TaskDev{};
+            T.m = a.nrows;
+            T.n = bb.ncols;
+            if (compress_mode) {
+                const int32_t k = a.ncols;
+                b->masks[t] = (uint64_t *)b->own(sizeof(uint64_t) * (size_t)(((int64_t)k + 63) / 64 + 1));
+                const int32_t rmax = std::max(a.nrows, bb.nrows);
+                b->kept_tmp[t] = (int32_t *)b->own(sizeof(int32_t) * ((size_t)rmax + 1));
+                b->np64[t] = (int64_t *)b->own(sizeof(int64_t) * ((size_t)rmax + 1));
+                b->scan_tmp[t] = (int64_t *)b->own(scan_tmp_bytes(rmax));
+                BatchOperand &ac = b->Ac[t], &bc = b->Bc[t];
+                ac.optr = (int32_t *)b->own(sizeof(int32_t) * ((size_t)a.nrows + 1));
+                ac.oidx = (int32_t *)b->own(sizeof(int32_t) * (size_t)std::max<int64_t>(a.nnz, 1));
+                ac.oval = (double *)b->own(sizeof(double) * (size_t)std::max<int64_t>(a.nnz, 1));
+                bc.optr = (int32_t *)b->own(sizeof(int32_t) * ((size_t)bb.nrows + 1));
+                bc.oidx = (int32_t *)b->own(sizeof(int32_t) * (size_t)std::max<int64_t>(bb.nnz, 1));
+                bc.oval = (double *)b->own(sizeof(double) * (size_t)std::max<int64_t>(bb.nnz, 1));
+                T.ap = ac.optr;
+                T.ai = ac.oidx;
+                T.av = ac.oval;
+                T.bp = bc.optr;
+                T.bi = bc.oidx;
+                T.bv = bc.oval;
+            } else {
+                T.ap = a.ptr_vec;
+                T.ai = a.idx_vec;
+                T.av = a.val;
+                T.bp = bb.ptr_vec;
+                T.bi = bb.idx_vec;
+                T.bv = bb.val;
+            }
+        }
+        b->ws_mul.reset(new Workspace(ctx, true));
+        b->ws_add.reset(new Workspace(ctx, true));
+        mul_setup(ctx, *b->ws_mul, b->mp, tasks);
+        // eager run: sizes every list and output
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        batch_body(b, false);
+        b->flops = (int64_t)b->mp.h.flops;
+        b->o_base.assign(ntasks, 0);
+        b->o_nnz.assign(ntasks, 0);
+        for (int t = 0; t < ntasks; ++t) {
+            const int32_t g = b->ap.h[t].row_base;
+            const int64_t cs = read_scalar<int64_t>(ctx, b->ap.cstart + g);
+            b->o_base[t] = read_scalar<int64_t>(ctx, b->ap.coff + cs);
+        }
+        for (int t = 0; t < ntasks; ++t)
+            b->o_nnz[t] = (t + 1 < ntasks ? b->o_base[t + 1] : b->o_total) - b->o_base[t];
+        for (int t = 0; t < ntasks; ++t)
+            if (b->o_nnz[t] > INT32_MAX) fail(TR_ERR_OVERFLOW, "batch: an accumulated tile exceeds INT32_MAX entries");
+        TR_HIP(hipStreamSynchronize(ctx->stream));
+        // capture the same sequence with fixed sizes
+        TR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            batch_body(b, true);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            ctx->timing = timing;
+            throw;
+        }
+        TR_HIP(hipStreamEndCapture(ctx->stream, &b->graph));
+        TR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        ctx->timing = timing;
+        b->ran = true;  // the eager pass already produced the results
+    });
+    if (rc != TR_OK) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return TR_OK;
+}
+
+int tr_batch_run(tr_batch_t b) {
+    if (!b) return TR_ERR_INVALID;
+    tr_ctx *ctx = b->ctx;
+    return bguard(ctx, [&] {
+        ::tr::TimerScope ts(ctx, "batch_graph");
+        TR_HIP(hipGraphLaunch(b->exec, ctx->stream));
+        b->ran = true;
+    });
+}
+
+int tr_batch_result(tr_batch_t b, int32_t t, tr_csx_t *C_out) {
+    if (!b) return TR_ERR_INVALID;
+    tr_ctx *ctx = b->ctx;
+    return bguard(ctx, [&] {
+        TR_REQUIRE(t >= 0 && t < b->nt && C_out, "tr_batch_result: bad task index");
+        tr_csx_t c{};
+        c.majordim = TR_ROWS;
+        c.nrows = b->A[t].nrows;
+        c.ncols = b->B[t].ncols;
+        c.nnz = b->o_nnz[t];
+        c.ptr_vec = b->o_ptr + b->optr_off[t];
+        c.idx_vec = b->o_idx + b->o_base[t];
+        c.val = b->o_val + b->o_base[t];
+        c.contig = 0;
+        *C_out = c;
+    });
+}
+
+int tr_batch_stats(tr_batch_t b, int64_t *flops, int64_t *nnz_products, int64_t *nnz_out) {
+    if (!b) return TR_ERR_INVALID;
+    if (flops) *flops = b->flops;
+    if (nnz_products) *nnz_products = b->p_total;
+    if (nnz_out) *nnz_out = b->o_total;
+    return TR_OK;
+}
+
+int tr_batch_destroy(tr_batch_t b) {
+    if (!b) return TR_ERR_INVALID;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return TR_OK;
+}
+
+}  // extern "C"

@@ -92,6 +92,20 @@ void launch_bitmask_and(tr_ctx *ctx, const uint64_t *a, const uint64_t *b, int32
     TR_LAUNCH(ctx, "bitmask_and", k_bitmask_and, div_up((uint64_t)words, 256), 256, 0, a, b, words, out);
 }
 
+// Segment compression into caller-provided arrays with capacity for the
+// input's nnz; no host round trip (usable inside a captured graph).
+void launch_compress_fixed(tr_ctx *ctx, int32_t n, const int32_t *ptr, const int32_t *idx, const double *val,
+                           const uint64_t *mask, int32_t *kept_tmp, int64_t *np64, int64_t *scan_tmp, int32_t *optr,
+                           int32_t *oidx, double *oval) {
+    if (n <= 0) return;
+    TR_LAUNCH(ctx, "compress_count", k_compress_count, div_up((uint64_t)n, 256), 256, 0, ptr, n, mask, kept_tmp);
+    launch_exclusive_scan_i32_to_i64(ctx, kept_tmp, np64, n, scan_tmp);
+    const int wpb = 4;
+    const uint32_t groups = div_up((uint64_t)n, kWave);
+    TR_LAUNCH(ctx, "compress_write", k_compress_write, div_up(groups, wpb), wpb * kWave, 0, ptr, idx, val, n, mask,
+              np64, optr, oidx, oval);
+}
+
 int64_t compress(tr_ctx *ctx, const tr_csx_t *m, const uint64_t *mask, tr_csx_t *out) {
     const int32_t n = ptrdim(m);
     int32_t *kept = (int32_t *)scratch(ctx, S_MISC, sizeof(int32_t) * ((size_t)n + 1));

@@ -1,4 +1,4 @@
-// Sorted sparse add C = A + B on gfx950 (partial-C accumulation).
+// Sorted sparse add C = X + Y on gfx950 (partial-C accumulation).
 //
 // Replaces the spadd of LocalMatrix::spadd / sp_mma
 // (include/KokkosWrap.hpp:341-352, :394-439: KokkosSparse spadd with
@@ -7,14 +7,20 @@
 //
 // Merge-path formulation: every row's union is cut into chunks of kChunk
 // merged positions; a wave owns a chunk and finds, per lane, the merge-path
-// split of its position by binary search (ties take A's entry first, so an
-// equal column appears as an adjacent (A, B) pair that collapses into one
-// output entry a + b).  Pass 1 counts the kept positions per chunk, one scan
-// over all chunks gives both the row pointer and every chunk's output offset,
-// pass 2 writes.  Long rows are spread over many waves; there is no host
-// round trip except the single read of the output size.
+// split of its position by binary search (ties take X's entry first, so an
+// equal column appears as an adjacent (X, Y) pair that collapses into one
+// output entry x + y).  Pass 1 counts the kept positions per chunk, one scan
+// over all chunks gives both the row pointers and every chunk's output
+// offset, pass 2 writes.  Long rows are spread over many waves.
+//
+// Like the multiply, the add runs over a table of tasks (a plain call is a
+// table of one; the batched stage executor adds all tile products into
+// their partial-C tiles in one pass).  X rows are addressed either by an
+// int32 pointer per task or by a 64-bit pointer over the global row space
+// (the product arena of a batch).
 #include "dev_common.h"
 #include "tr_internal.h"
+#include "spgemm_types.h"
 
 namespace tr {
 namespace {
@@ -23,120 +29,208 @@ using namespace dev;
 
 constexpr int kChunk = 256;
 
-__global__ void k_add_chunks(const int32_t *__restrict__ ap, const int32_t *__restrict__ bp, int32_t m,
-                             int32_t *__restrict__ nch) {
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    int64_t len = (int64_t)(ap[i + 1] - ap[i]) + (int64_t)(bp[i + 1] - bp[i]);
-    nch[i] = (int32_t)((len + kChunk - 1) / kChunk);
-}
-
-__global__ void k_add_chunk_rows(const int64_t *__restrict__ cstart, int32_t m, int32_t *__restrict__ crow) {
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    for (int64_t c = cstart[i]; c < cstart[i + 1]; ++c) crow[c] = i;
-}
-
-// merged element at position t of row (A_i, B_i); returns key, value and
-// whether the position is kept (a B entry equal to the preceding A entry is
-// folded into it)
-__device__ __forceinline__ bool merge_at(const int32_t *__restrict__ ai, const double *__restrict__ av, int32_t la,
-                                         const int32_t *__restrict__ bi, const double *__restrict__ bv, int32_t lb,
-                                         int32_t t, int32_t &key, double &val, bool want_val) {
-    int32_t lo = t - lb > 0 ? t - lb : 0;
-    int32_t hi = t < la ? t : la;
+__device__ __forceinline__ int add_task_of(const AddTaskDev *__restrict__ tasks, int nt, int32_t g) {
+    int lo = 0, hi = nt - 1;
     while (lo < hi) {
-        int32_t mid = (lo + hi) >> 1;
-        if (ai[mid] <= bi[t - 1 - mid]) lo = mid + 1;
+        const int mid = (lo + hi + 1) >> 1;
+        if (tasks[mid].row_base <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct RowSpan {
+    int64_t xs, ys;
+    int32_t lx, ly;
+};
+
+__device__ __forceinline__ RowSpan row_span(const AddTaskDev &T, int32_t g) {
+    const int32_t r = g - T.row_base;
+    RowSpan s;
+    if (T.xp64) {
+        s.xs = T.xp64[g];
+        s.lx = (int32_t)(T.xp64[g + 1] - s.xs);
+    } else {
+        s.xs = T.xp[r];
+        s.lx = T.xp[r + 1] - T.xp[r];
+    }
+    s.ys = T.yp[r];
+    s.ly = T.yp[r + 1] - T.yp[r];
+    return s;
+}
+
+__global__ void k_add_chunks(const AddTaskDev *__restrict__ tasks, int nt, int32_t rows, int32_t *__restrict__ nch) {
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= rows) return;
+    const RowSpan s = row_span(tasks[add_task_of(tasks, nt, g)], g);
+    const int64_t len = (int64_t)s.lx + (int64_t)s.ly;
+    nch[g] = (int32_t)((len + kChunk - 1) / kChunk);
+}
+
+__global__ void k_add_chunk_rows(const int64_t *__restrict__ cstart, int32_t rows, int32_t *__restrict__ crow) {
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= rows) return;
+    for (int64_t c = cstart[g]; c < cstart[g + 1]; ++c) crow[c] = g;
+}
+
+// merged element at position t of a row (X_i, Y_i); returns key, value and
+// whether the position is kept (a Y entry equal to the preceding X entry is
+// folded into it)
+__device__ __forceinline__ bool merge_at(const int32_t *__restrict__ xi, const double *__restrict__ xv, int32_t lx,
+                                         const int32_t *__restrict__ yi, const double *__restrict__ yv, int32_t ly,
+                                         int32_t t, int32_t &key, double &val, bool want_val) {
+    int32_t lo = t - ly > 0 ? t - ly : 0;
+    int32_t hi = t < lx ? t : lx;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (xi[mid] <= yi[t - 1 - mid]) lo = mid + 1;
         else hi = mid;
     }
     const int32_t i = lo, j = t - lo;
-    bool fromA = (i < la) && (j >= lb || ai[i] <= bi[j]);
-    if (fromA) {
-        key = ai[i];
+    const bool fromX = (i < lx) && (j >= ly || xi[i] <= yi[j]);
+    if (fromX) {
+        key = xi[i];
         if (want_val) {
-            val = av[i];
-            if (j < lb && bi[j] == key) val = val + bv[j];
+            val = xv[i];
+            if (j < ly && yi[j] == key) val = val + yv[j];
         }
         return true;
     }
-    key = bi[j];
-    if (i > 0 && ai[i - 1] == key) return false;  // folded into the A entry
-    if (want_val) val = bv[j];
+    key = yi[j];
+    if (i > 0 && xi[i - 1] == key) return false;  // folded into the X entry
+    if (want_val) val = yv[j];
     return true;
 }
 
+// chunk count is read on the device (cstart[rows]); grid-stride so the
+// launch can be replayed from a graph
 template <bool kWrite>
-__global__ __launch_bounds__(256) void k_add_pass(const int32_t *__restrict__ ap, const int32_t *__restrict__ ai,
-                                                  const double *__restrict__ av, const int32_t *__restrict__ bp,
-                                                  const int32_t *__restrict__ bi, const double *__restrict__ bv,
+__global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__ tasks, int nt, int32_t rows,
                                                   const int64_t *__restrict__ cstart,
-                                                  const int32_t *__restrict__ crow, int64_t nchunks,
-                                                  int32_t *__restrict__ ccount, const int64_t *__restrict__ coff,
-                                                  int32_t *__restrict__ oi, double *__restrict__ ov) {
+                                                  const int32_t *__restrict__ crow, int32_t *__restrict__ ccount,
+                                                  const int64_t *__restrict__ coff, int32_t *__restrict__ oi,
+                                                  double *__restrict__ ov) {
     const int l = lane_id();
-    const int64_t c = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave;
-    if (c >= nchunks) return;
-    const int32_t row = crow[c];
-    const int32_t a0 = ap[row], la = ap[row + 1] - a0;
-    const int32_t b0 = bp[row], lb = bp[row + 1] - b0;
-    const int32_t t0 = (int32_t)(c - cstart[row]) * kChunk;
-    const int32_t tend = min(t0 + kChunk, la + lb);
-    int64_t out = kWrite ? coff[c] : 0;
-    int32_t kept_total = 0;
-    for (int32_t tb = t0; tb < tend; tb += kWave) {
-        const int32_t t = tb + l;
-        int32_t key = 0;
-        double val = 0.0;
-        bool keep = false;
-        if (t < tend) keep = merge_at(ai + a0, av + a0, la, bi + b0, bv + b0, lb, t, key, val, kWrite);
-        const uint64_t msk = ballot(keep);
-        if (kWrite && keep) {
-            const int64_t q = out + rank_below(msk);
-            oi[q] = key;
-            ov[q] = val;
+    const int64_t nchunks = cstart[rows];
+    for (int64_t c = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave; c < nchunks; c += (int64_t)gridDim.x * 4) {
+        const int32_t g = crow[c];
+        const AddTaskDev &T = tasks[add_task_of(tasks, nt, g)];
+        const RowSpan s = row_span(T, g);
+        const int32_t *__restrict__ xi = T.xi + s.xs;
+        const double *__restrict__ xv = T.xv + s.xs;
+        const int32_t *__restrict__ yi = T.yi + s.ys;
+        const double *__restrict__ yv = T.yv + s.ys;
+        const int32_t t0 = (int32_t)(c - cstart[g]) * kChunk;
+        const int32_t tend = min(t0 + kChunk, s.lx + s.ly);
+        int64_t out = kWrite ? coff[c] : 0;
+        int32_t kept_total = 0;
+        for (int32_t tb = t0; tb < tend; tb += kWave) {
+            const int32_t t = tb + l;
+            int32_t key = 0;
+            double val = 0.0;
+            bool keep = false;
+            if (t < tend) keep = merge_at(xi, xv, s.lx, yi, yv, s.ly, t, key, val, kWrite);
+            const uint64_t msk = ballot(keep);
+            if (kWrite && keep) {
+                const int64_t q = out + rank_below(msk);
+                oi[q] = key;
+                ov[q] = val;
+            }
+            out += __popcll(msk);
+            kept_total += __popcll(msk);
         }
-        out += __popcll(msk);
-        kept_total += __popcll(msk);
+        if (!kWrite && l == 0) ccount[c] = kept_total;
     }
-    if (!kWrite && l == 0) ccount[c] = kept_total;
 }
 
-__global__ void k_add_rowptr(const int64_t *__restrict__ cstart, const int64_t *__restrict__ coff, int32_t m,
-                             int32_t *__restrict__ optr) {
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > m) return;
-    optr[i] = (int32_t)coff[cstart[i]];
+// int32 row pointers of every task's output, rebased to the task
+__global__ void k_add_rowptr(const AddTaskDev *__restrict__ tasks, int nt, int32_t rows,
+                             const int64_t *__restrict__ cstart, const int64_t *__restrict__ coff) {
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < rows) {
+        const AddTaskDev &T = tasks[add_task_of(tasks, nt, g)];
+        const int64_t base = coff[cstart[T.row_base]];
+        const int32_t r = g - T.row_base;
+        T.optr[r] = (int32_t)(coff[cstart[g]] - base);
+        if (r == T.m - 1) T.optr[T.m] = (int32_t)(coff[cstart[g + 1]] - base);
+    }
+    if (g < nt && tasks[g].m == 0) tasks[g].optr[0] = 0;
 }
 
 }  // namespace
 
+void add_setup(tr_ctx *ctx, Workspace &ws, AddPlan &ap, const std::vector<AddTaskDev> &tasks, int64_t nnz_x,
+               int64_t nnz_y) {
+    ap.nt = (int)tasks.size();
+    ap.h = tasks;
+    int64_t rows = 0;
+    for (auto &t : ap.h) {
+        t.row_base = (int32_t)rows;
+        rows += t.m;
+    }
+    TR_REQUIRE(rows < INT32_MAX, "spadd: total rows exceed INT32_MAX");
+    ap.rows_total = (int32_t)rows;
+    ap.chunk_cap = rows + (nnz_x + nnz_y) / kChunk + 1;
+    const size_t R = (size_t)rows + 1;
+    ap.d = (AddTaskDev *)ws.get(S_PANELS, sizeof(AddTaskDev) * (size_t)std::max(ap.nt, 1));
+    ap.nch = (int32_t *)ws.get(S_MISC, sizeof(int32_t) * R);
+    ap.cstart = (int64_t *)ws.get(S_ROWPTR64, sizeof(int64_t) * R);
+    ap.crow = (int32_t *)ws.get(S_BINS, sizeof(int32_t) * (size_t)ap.chunk_cap);
+    ap.ccount = (int32_t *)ws.get(S_MISC2, sizeof(int32_t) * ((size_t)ap.chunk_cap + 1));
+    ap.coff = (int64_t *)ws.get(S_FLOPS, sizeof(int64_t) * ((size_t)ap.chunk_cap + 1));
+    if (ws.owned)
+        ap.scan_tmp = (int64_t *)ws.get(S_SCAN_TMP, scan_tmp_bytes(std::max<int64_t>(ap.chunk_cap, rows)));
+    TR_HIP(hipMemcpyAsync(ap.d, ap.h.data(), sizeof(AddTaskDev) * (size_t)ap.nt, hipMemcpyHostToDevice,
+                          ctx->stream));
+}
+
+void add_count(tr_ctx *ctx, AddPlan &ap, bool capture) {
+    const int32_t rows = ap.rows_total;
+    TR_LAUNCH(ctx, "add_chunks", k_add_chunks, div_up((uint64_t)rows, 256), 256, 0, ap.d, ap.nt, rows, ap.nch);
+    launch_exclusive_scan_i32_to_i64(ctx, ap.nch, ap.cstart, rows, ap.scan_tmp);
+    TR_LAUNCH(ctx, "add_chunk_rows", k_add_chunk_rows, div_up((uint64_t)rows, 256), 256, 0, ap.cstart, rows, ap.crow);
+    if (!capture) {
+        ap.nchunks = read_scalar<int64_t>(ctx, ap.cstart + rows);
+        TR_REQUIRE(ap.nchunks <= ap.chunk_cap, "spadd: chunk bound violated");
+    }
+    TR_LAUNCH(ctx, "add_count", k_add_pass<false>, div_up((uint64_t)std::max<int64_t>(ap.nchunks, 1), 4), 256, 0,
+              ap.d, ap.nt, rows, ap.cstart, ap.crow, ap.ccount, (const int64_t *)nullptr, (int32_t *)nullptr,
+              (double *)nullptr);
+    launch_exclusive_scan_i32_to_i64(ctx, ap.ccount, ap.coff, ap.nchunks, ap.scan_tmp);
+}
+
+int64_t add_total(tr_ctx *ctx, AddPlan &ap) { return read_scalar<int64_t>(ctx, ap.coff + ap.nchunks); }
+
+void add_write(tr_ctx *ctx, AddPlan &ap, int32_t *oi, double *ov) {
+    const int32_t rows = ap.rows_total;
+    TR_LAUNCH(ctx, "add_rowptr", k_add_rowptr, div_up((uint64_t)std::max<int32_t>(rows, ap.nt), 256), 256, 0, ap.d,
+              ap.nt, rows, ap.cstart, ap.coff);
+    TR_LAUNCH(ctx, "add_write", k_add_pass<true>, div_up((uint64_t)std::max<int64_t>(ap.nchunks, 1), 4), 256, 0,
+              ap.d, ap.nt, rows, ap.cstart, ap.crow, (int32_t *)nullptr, ap.coff, oi, ov);
+}
+
 // out = A + C (both CSR, sorted), out context-owned.
 void spadd(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, tr_csx_t *out) {
-    const int32_t m = A->nrows;
-    int32_t *nch = (int32_t *)scratch(ctx, S_MISC, sizeof(int32_t) * ((size_t)m + 1));
-    int64_t *cstart = (int64_t *)scratch(ctx, S_ROWPTR64, sizeof(int64_t) * ((size_t)m + 1));
-    TR_LAUNCH(ctx, "add_chunks", k_add_chunks, div_up((uint64_t)m, 256), 256, 0, A->ptr_vec, C->ptr_vec, m, nch);
-    launch_exclusive_scan_i32_to_i64(ctx, nch, cstart, m);
-    // chunk count bound: m + (nnzA + nnzC) / kChunk
-    const int64_t cap = (int64_t)m + (A->nnz + C->nnz) / kChunk + 1;
-    int32_t *crow = (int32_t *)scratch(ctx, S_BINS, sizeof(int32_t) * (size_t)cap);
-    int32_t *ccount = (int32_t *)scratch(ctx, S_MISC2, sizeof(int32_t) * ((size_t)cap + 1));
-    int64_t *coff = (int64_t *)scratch(ctx, S_FLOPS, sizeof(int64_t) * ((size_t)cap + 1));
-    TR_LAUNCH(ctx, "add_chunk_rows", k_add_chunk_rows, div_up((uint64_t)m, 256), 256, 0, cstart, m, crow);
-    const int64_t nchunks = read_scalar<int64_t>(ctx, cstart + m);
-    TR_REQUIRE(nchunks <= cap, "spadd: chunk bound violated");
-    TR_LAUNCH(ctx, "add_count", k_add_pass<false>, div_up((uint64_t)nchunks, 4), 256, 0, A->ptr_vec, A->idx_vec,
-              A->val, C->ptr_vec, C->idx_vec, C->val, cstart, crow, nchunks, ccount, (const int64_t *)nullptr,
-              (int32_t *)nullptr, (double *)nullptr);
-    launch_exclusive_scan_i32_to_i64(ctx, ccount, coff, nchunks);
-    const int64_t nnz = read_scalar<int64_t>(ctx, coff + nchunks);
+    Workspace ws(ctx, false);
+    AddPlan ap;
+    AddTaskDev t{};
+    t.xp = A->ptr_vec;
+    t.xi = A->idx_vec;
+    t.xv = A->val;
+    t.yp = C->ptr_vec;
+    t.yi = C->idx_vec;
+    t.yv = C->val;
+    t.m = A->nrows;
+    // the output row pointer is written into the tile allocated below;
+    // point the task at it after allocation
+    add_setup(ctx, ws, ap, {t}, A->nnz, C->nnz);
+    add_count(ctx, ap, false);
+    const int64_t nnz = add_total(ctx, ap);
     if (nnz > INT32_MAX) fail(TR_ERR_OVERFLOW, "spadd: nnz(A + C) exceeds INT32_MAX");
-    alloc_tile(ctx, TR_ROWS, m, A->ncols, nnz, out);
-    TR_LAUNCH(ctx, "add_rowptr", k_add_rowptr, div_up((uint64_t)m + 1, 256), 256, 0, cstart, coff, m, out->ptr_vec);
-    TR_LAUNCH(ctx, "add_write", k_add_pass<true>, div_up((uint64_t)nchunks, 4), 256, 0, A->ptr_vec, A->idx_vec,
-              A->val, C->ptr_vec, C->idx_vec, C->val, cstart, crow, nchunks, (int32_t *)nullptr, coff, out->idx_vec,
-              out->val);
+    alloc_tile(ctx, TR_ROWS, A->nrows, A->ncols, nnz, out);
+    ap.h[0].optr = out->ptr_vec;
+    TR_HIP(hipMemcpyAsync(ap.d, ap.h.data(), sizeof(AddTaskDev), hipMemcpyHostToDevice, ctx->stream));
+    add_write(ctx, ap, out->idx_vec, out->val);
 }
 
 }  // namespace tr

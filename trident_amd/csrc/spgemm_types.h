@@ -113,6 +113,38 @@ struct MulPlan {
     size_t scan_tmp_bytes = 0;
 };
 
+// One add X_t + Y_t of a batch.  X rows come from xp (int32, per task) or,
+// when xp64 is set, from a 64-bit pointer over the global row space.
+struct AddTaskDev {
+    const int32_t *xp;
+    const int64_t *xp64;
+    const int32_t *xi;
+    const double *xv;
+    const int32_t *yp;
+    const int32_t *yi;
+    const double *yv;
+    int32_t *optr;     // output row pointer (m + 1), rebased to the task
+    int32_t m;
+    int32_t row_base;
+};
+
+struct AddPlan {
+    int nt = 0;
+    int32_t rows_total = 0;
+    std::vector<AddTaskDev> h;
+    AddTaskDev *d = nullptr;
+    int32_t *nch = nullptr, *crow = nullptr, *ccount = nullptr;
+    int64_t *cstart = nullptr, *coff = nullptr;
+    int64_t chunk_cap = 0, nchunks = 0;
+    int64_t *scan_tmp = nullptr;
+};
+
+void add_setup(tr_ctx *ctx, Workspace &ws, AddPlan &ap, const std::vector<AddTaskDev> &tasks, int64_t nnz_x,
+               int64_t nnz_y);
+void add_count(tr_ctx *ctx, AddPlan &ap, bool capture);
+int64_t add_total(tr_ctx *ctx, AddPlan &ap);
+void add_write(tr_ctx *ctx, AddPlan &ap, int32_t *oi, double *ov);
+
 // Pipeline steps.  In eager mode the host reads counts between steps to size
 // grids and lists; in capture mode every size comes from a previous eager run
 // on the same operands.

@@ -169,6 +169,9 @@ void launch_rowptr_i64_to_i32(tr_ctx *ctx, const int64_t *rp64, int32_t r0, int3
 void launch_bitmask_gen(tr_ctx *ctx, const int32_t *ptr, int32_t n, uint64_t *mask);
 void launch_bitmask_and(tr_ctx *ctx, const uint64_t *a, const uint64_t *b, int32_t words, uint64_t *out);
 int64_t compress(tr_ctx *ctx, const tr_csx_t *m, const uint64_t *mask, tr_csx_t *out);
+void launch_compress_fixed(tr_ctx *ctx, int32_t n, const int32_t *ptr, const int32_t *idx, const double *val,
+                           const uint64_t *mask, int32_t *kept_tmp, int64_t *np64, int64_t *scan_tmp, int32_t *optr,
+                           int32_t *oidx, double *oval);
 
 // k_spgemm.hip
 void validate_operands(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B);

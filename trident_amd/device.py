@@ -296,3 +296,49 @@ class SpaComm:
         ctx.check(ctx.lib.tr_compress(ctx.h, ctypes.byref(M.d), ctypes.c_void_p(mask_dev), ctypes.byref(out)),
                   "tr_compress")
         return DeviceCsx(ctx, out)
+
+
+class Batch:
+    """One stage of tile products as a single hipGraph launch
+    (tr_batch_create / tr_batch_run): C_out_t = C_in_t + A_t * B_t."""
+
+    def __init__(self, ctx: Context, A: List[DeviceCsx], B: List[DeviceCsx],
+                 C_in: Optional[List[Optional[DeviceCsx]]] = None, compress: bool = False):
+        n = len(A)
+        assert len(B) == n and (C_in is None or len(C_in) == n)
+        self.ctx = ctx
+        self.n = n
+        arrA = (tr_csx_t * n)(*[a.d for a in A])
+        arrB = (tr_csx_t * n)(*[b.d for b in B])
+        arrC = None
+        if C_in is not None:
+            arrC = (tr_csx_t * n)(*[(c.d if c is not None else tr_csx_t()) for c in C_in])
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.tr_batch_create(ctx.h, n, arrA, arrB, arrC, 1 if compress else 0, ctypes.byref(h)),
+                  "tr_batch_create")
+        self.h = h
+        self._keep = (A, B, C_in)  # operands must outlive the batch
+
+    def run(self):
+        self.ctx.check(self.ctx.lib.tr_batch_run(self.h), "tr_batch_run")
+
+    def result(self, t: int) -> DeviceCsx:
+        d = tr_csx_t()
+        self.ctx.check(self.ctx.lib.tr_batch_result(self.h, t, ctypes.byref(d)), "tr_batch_result")
+        return DeviceCsx(self.ctx, d)
+
+    def stats(self) -> Tuple[int, int, int]:
+        f, p, o = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self.ctx.lib.tr_batch_stats(self.h, ctypes.byref(f), ctypes.byref(p), ctypes.byref(o))
+        return int(f.value), int(p.value), int(o.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.tr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
