@@ -120,6 +120,273 @@ def cpu_baseline_and_check(A, f, ctx, panels, rb, hptrs, target_mults, seed=7):
     }, {"rows_checked": int(len(rows)), "structure_equal": same_struct, "max_rel_err": max_rel}
 
 
+def stage_bench(args):
+    """Batched stage (R-MAT scale S, 4x4 tiles, SUMMA stage k): the 16 tile
+    products C_ij <- C_ij + A_ik * A_kj as one graph launch vs 16 plain
+    sp_mma calls.  Partial-C tiles hold the same stage's products from a
+    previous pass, so the accumulate merges equal structures."""
+    from trident_amd import gen
+    from trident_amd.device import Batch, Context, DeviceCsx, sp_mma, spgemm
+    _, scale, k = args.workload.split(":")
+    scale, k = int(scale), int(k)
+    g = 4
+    t0 = time.perf_counter()
+    A = gen.rmat(scale, 16, seed=2)
+    n = A.nrows // g
+    tiles = {}
+    for i in range(g):
+        tiles[(i, k)] = gen.submatrix(A, i * n, (i + 1) * n, k * n, (k + 1) * n)
+        tiles[(k, i)] = gen.submatrix(A, k * n, (k + 1) * n, i * n, (i + 1) * n)
+    log(f"[stage] tiles ready in {time.perf_counter() - t0:.1f}s")
+    ctx = Context(0)
+    dT = {key: ctx.upload(m) for key, m in tiles.items()}
+    keys = [(i, j) for i in range(g) for j in range(g)]
+    Cin = [spgemm(dT[(i, k)], dT[(k, j)]) for i, j in keys]
+    ctx.synchronize()
+    from trident_amd.device import spgemm_flops
+    mults = sum(spgemm_flops(dT[(i, k)], dT[(k, j)]) for i, j in keys)
+    log(f"[stage] stage {k}: {mults:.4e} multiplications, partial C {sum(c.nnz for c in Cin):.4e} entries")
+
+    def plain_step():
+        outs = []
+        for t, (i, j) in enumerate(keys):
+            C = DeviceCsx(ctx, type(Cin[t].d).from_buffer_copy(Cin[t].d))
+            C.d.contig = 0  # borrowed view: sp_mma replaces it, the original stays
+            C.d.buf = None
+            sp_mma(dT[(i, k)], dT[(k, j)], C)
+            outs.append(C)
+        ctx.synchronize()
+        return outs
+
+    res = {}
+    for mode in ("plain", "batched", "batched+compress"):
+        if mode == "plain":
+            outs = plain_step()
+            for C in outs:
+                C.free()
+            ts = []
+            for _ in range(args.steps):
+                t = time.perf_counter()
+                outs = plain_step()
+                ts.append(time.perf_counter() - t)
+                if _ + 1 < args.steps:
+                    for C in outs:
+                        C.free()
+            plain_outs = outs
+        else:
+            b = Batch(ctx, [dT[(i, k)] for i, j in keys], [dT[(k, j)] for i, j in keys], Cin,
+                      compress=(mode == "batched+compress"))
+            b.run()
+            ctx.synchronize()
+            ts = []
+            for _ in range(args.steps):
+                t = time.perf_counter()
+                b.run()
+                ctx.synchronize()
+                ts.append(time.perf_counter() - t)
+            # parity: batched tiles == plain tiles on sampled rows
+            rng = np.random.default_rng(1)
+            ok = True
+            for tt in range(len(keys)):
+                Hb, Hp = b.result(tt), plain_outs[tt]
+                pb = ctx.to_host(Hb.d.ptr_vec, (Hb.d.nrows + 1,), np.int32)
+                pp = ctx.to_host(Hp.d.ptr_vec, (Hp.d.nrows + 1,), np.int32)
+                ok &= bool(np.array_equal(pb, pp))
+                for r in rng.choice(Hb.d.nrows, 20, replace=False):
+                    s0, e0 = int(pb[r]), int(pb[r + 1])
+                    if e0 > s0:
+                        a1 = ctx.to_host(Hb.d.val + 8 * s0, (e0 - s0,), np.float64)
+                        a2 = ctx.to_host(Hp.d.val + 8 * s0, (e0 - s0,), np.float64)
+                        ok &= bool(np.array_equal(a1, a2))
+            res[mode + "_matches_plain"] = ok
+            b.close()
+        t_med = float(np.median(ts))
+        res[mode] = {"ms": round(1000 * t_med, 2), "GFLOP/s": round(2 * mults / t_med / 1e9, 2)}
+        log(f"[stage] {mode}: {res[mode]}")
+    res["speedup_batched_vs_plain"] = round(res["plain"]["ms"] / res["batched"]["ms"], 3)
+    out = {
+        "metric": METRIC, "value": res["batched"]["GFLOP/s"], "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": res["batched"]["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"R-MAT scale {scale} (ef 16, seed 2) cut 4x4; SUMMA stage k={k}: 16 tile products "
+                               "accumulated into partial-C tiles", "mults": mults, "forms": res},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def galerkin_bench(args):
+    """AMG Galerkin product C = P^T (A P): 27-point Laplacian on n^3 (26/-1),
+    smoothed-aggregation P (3^3 aggregates, damped Jacobi w = 2/3).  Plain:
+    two tr_spgemm calls; batched: the two products as captured graphs (P^T
+    from the on-device transpose, resident)."""
+    from trident_amd import gen
+    from trident_amd.device import Batch, Context, spgemm, spgemm_flops
+    n = int(args.workload.split(":")[1])
+    t0 = time.perf_counter()
+    A = gen.laplace27(n)
+    P = gen.sa_prolongator(n)
+    log(f"[galerkin] operators ready in {time.perf_counter() - t0:.1f}s: A {A.nrows} x {A.ncols} nnz {A.nnz}, "
+        f"P {P.nrows} x {P.ncols} nnz {P.nnz}")
+    ctx = Context(0)
+    dA, dP = ctx.upload(A), ctx.upload(P)
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    t = time.perf_counter()
+    dPt = dP.transpose()
+    ctx.synchronize()
+    t_tr = time.perf_counter() - t
+    tr_kernels = {k: round(v[1], 3) for k, v in ctx.timings().items()}
+    ctx.set_timing(False)
+    AP = spgemm(dA, dP)
+    mults = spgemm_flops(dA, dP) + spgemm_flops(dPt, AP)
+    C = spgemm(dPt, AP)
+    ctx.synchronize()
+    ref = C.download()
+
+    def plain():
+        ap = spgemm(dA, dP)
+        c = spgemm(dPt, ap)
+        ctx.synchronize()
+        return ap, c
+    ap, c = plain()
+    ts = []
+    for _ in range(args.steps):
+        ap.free(), c.free()
+        t = time.perf_counter()
+        ap, c = plain()
+        ts.append(time.perf_counter() - t)
+    t_plain = float(np.median(ts))
+    b1 = Batch(ctx, [dA], [dP])
+    ap_b = b1.result(0)
+    b2 = Batch(ctx, [dPt], [ap_b])
+    ts = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        b1.run()
+        b2.run()
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t)
+    t_b = float(np.median(ts))
+    got = b2.result(0).download()
+    ok = bool(np.array_equal(got.ptr, ref.ptr) and np.array_equal(got.idx, ref.idx) and np.array_equal(got.val, ref.val))
+    res = {"transpose_ms": round(1000 * t_tr, 3), "transpose_kernels_ms": tr_kernels,
+           "plain": {"ms": round(1000 * t_plain, 3), "GFLOP/s": round(2 * mults / t_plain / 1e9, 2)},
+           "batched": {"ms": round(1000 * t_b, 3), "GFLOP/s": round(2 * mults / t_b / 1e9, 2)},
+           "batched_matches_plain": ok, "nnz_AP": ap_b.nnz, "nnz_C": got.nnz}
+    log(f"[galerkin] {res}")
+    out = {"metric": METRIC, "value": res["batched"]["GFLOP/s"], "unit": "GFLOP/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": 1, "ms_per_step": res["batched"]["ms"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"C = P^T A P, A = 27-point Laplacian {n}^3 (26/-1), P = SA prolongator "
+                                  "(3^3 aggregates, unnormalised tentative P, w = 2/3)", "mults": mults, "forms": res}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def serve_bench(args):
+    """Served stream of mixed requests with Poisson arrivals: 60% A*A on R-MAT
+    scales 12-18 (edge factor 8 or 16), 40% P^T A P on 27-point Laplacians
+    16^3-96^3.  Reports p50/p99 latency and throughput per arrival rate, and
+    the plain aggregate (each request alone, one after another)."""
+    import threading
+    from trident_amd import gen
+    from trident_amd.device import (Context, Server, TR_REQ_GALERKIN, TR_REQ_SPGEMM, spgemm, spgemm_flops)
+    parts = args.workload.split(":")
+    nreq = int(parts[1]) if len(parts) > 1 else 2000
+    rng = np.random.default_rng(3)
+    t0 = time.perf_counter()
+    ctx = Context(0)
+    rm = {}
+    for s_ in range(12, 19):
+        for ef in (8, 16):
+            rm[(s_, ef)] = ctx.upload(gen.rmat(s_, ef, seed=100 * s_ + ef))
+    lp = {}
+    for n in (16, 32, 48, 64, 80, 96):
+        lp[n] = (ctx.upload(gen.laplace27(n)), ctx.upload(gen.sa_prolongator(n)))
+    log(f"[serve] operands ready in {time.perf_counter() - t0:.1f}s")
+    # plain reference: every request type alone, one after another
+    kinds = [("spgemm", k) for k in rm] + [("galerkin", n) for n in lp]
+    plain_t, flops = {}, {}
+    for kind, key in kinds:
+        for rep in range(2):
+            t = time.perf_counter()
+            if kind == "spgemm":
+                c = spgemm(rm[key], rm[key])
+                ctx.synchronize()
+                dt = time.perf_counter() - t
+                if rep == 1:
+                    flops[(kind, key)] = spgemm_flops(rm[key], rm[key])
+                c.free()
+            else:
+                A, P = lp[key]
+                Pt = P.transpose()
+                ctx.synchronize()
+                t = time.perf_counter()
+                ap = spgemm(A, P)
+                c = spgemm(Pt, ap)
+                ctx.synchronize()
+                dt = time.perf_counter() - t
+                if rep == 1:
+                    flops[(kind, key)] = spgemm_flops(A, P) + spgemm_flops(Pt, ap)
+                ap.free(), c.free(), Pt.free()
+        plain_t[(kind, key)] = dt
+    reqs = []
+    for _ in range(nreq):
+        if rng.random() < 0.6:
+            reqs.append(("spgemm", (int(rng.integers(12, 19)), int(rng.choice([8, 16])))))
+        else:
+            reqs.append(("galerkin", int(rng.choice(list(lp)))))
+    tot_flops = sum(flops[r] for r in reqs)
+    tot_plain = sum(plain_t[r] for r in reqs)
+    plain_gflops = 2 * tot_flops / tot_plain / 1e9
+    log(f"[serve] plain: {plain_gflops:.2f} GFLOP/s over the mix ({tot_plain:.1f} s sequential)")
+    srv = Server(mem_budget=int(args.serve_budget_gb * (1 << 30)), workers=args.serve_workers, max_batch=32)
+    ops = {k: srv.register(v) for k, v in rm.items()}
+    lops = {n: (srv.register(A), srv.register(P)) for n, (A, P) in lp.items()}
+    capacity = nreq / tot_plain  # requests/s the plain path sustains
+    results = []
+    for factor in args.serve_rates:
+        rate = factor * capacity
+        gaps = rng.exponential(1.0 / rate, size=nreq)
+        srv.reset_stats()
+        ids = []
+        t_start = time.perf_counter()
+        t_next = t_start
+        for i, (kind, key) in enumerate(reqs):
+            t_next += gaps[i]
+            now = time.perf_counter()
+            if t_next > now:
+                time.sleep(t_next - now)
+            if kind == "spgemm":
+                ids.append(srv.submit(TR_REQ_SPGEMM, ops[key], ops[key], discard=True))
+            else:
+                ids.append(srv.submit(TR_REQ_GALERKIN, lops[key][0], lops[key][1], discard=True))
+            if i % 500 == 0:
+                log(f"[serve] rate x{factor}: submitted {i}/{nreq}")
+        for rid in ids:
+            srv.wait(rid, want_result=False)
+            srv.release(rid)
+        wall = time.perf_counter() - t_start
+        st = srv.stats()
+        st.update({"rate_x_plain_capacity": factor, "arrival_rate_per_s": round(rate, 2),
+                   "wall_s": round(wall, 2), "served_GFLOP/s": round(2 * tot_flops / wall / 1e9, 2),
+                   "vs_plain": round((2 * tot_flops / wall / 1e9) / plain_gflops, 3)})
+        st = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
+        log(f"[serve] {st}")
+        results.append(st)
+    srv.close()
+    best = max(results, key=lambda r: r["served_GFLOP/s"])
+    out = {"metric": METRIC, "value": best["served_GFLOP/s"], "unit": "GFLOP/s", "n_gpus": 1, "steps": nreq,
+           "warmup": 0, "ms_per_step": round(1000 * best["wall_s"] / nreq, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{nreq} requests, Poisson arrivals: 60% A*A on R-MAT s12-18 (ef 8/16), 40% "
+                                  "P^T A P on 27-pt Laplacians 16^3-96^3", "plain_GFLOP/s": round(plain_gflops, 2),
+                      "rates": results}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,11 +396,20 @@ def main():
     ap.add_argument("--cpu-mults", type=float, default=1.5e9,
                     help="multiplications in the CPU-baseline row sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-budget-gb", type=float, default=160.0)
+    ap.add_argument("--serve-workers", type=int, default=3)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.5, 1.0, 4.0])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU's share of the path; --gpus {args.gpus} is not supported "
               "(run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.workload.startswith("stage:"):
+        return stage_bench(args)
+    if args.workload.startswith("galerkin:"):
+        return galerkin_bench(args)
+    if args.workload.startswith("serve"):
+        return serve_bench(args)
 
     from trident_amd.device import Context, spgemm_panels
 

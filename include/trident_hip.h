@@ -134,22 +134,58 @@ int tr_spgemm_flops(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t 
  *  - A[t], B[t]: CSR tiles (borrowed, sorted), A[t].ncols == B[t].nrows;
  *  - C_in: NULL or ntasks accumulators (borrowed; ptr_vec == NULL entries
  *    are empty);
- *  - compress_mode 0: no compression; 1: bitmask compression with
+ *  - flags: TR_BATCH_COMPRESS applies bitmask compression with
  *    mask = non-empty columns of A[t] AND non-empty rows of B[t] (the filter
- *    of SpaComm, include/spacomm.cuh:165-352, :962-1091).
+ *    of SpaComm, include/spacomm.cuh:165-352, :962-1091); TR_BATCH_NO_GRAPH
+ *    runs the batch once without capturing it (one-shot batches, e.g. the
+ *    serving loop).
  * tr_batch_create plans the stage (one eager pass that sizes every buffer
  * and already produces the results) and captures it; tr_batch_run replays
  * the graph on the context stream with no host round trip.  Inputs must keep
  * their shapes and addresses between create and run.  Results are
  * batch-owned (tr_batch_result) and stay valid until the next run or
  * tr_batch_destroy. */
+#define TR_BATCH_COMPRESS 1
+#define TR_BATCH_NO_GRAPH 2
 typedef struct tr_batch *tr_batch_t;
 int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_csx_t *B, const tr_csx_t *C_in,
-                    int32_t compress_mode, tr_batch_t *batch);
+                    int32_t flags, tr_batch_t *batch);
 int tr_batch_run(tr_batch_t batch);
 int tr_batch_result(tr_batch_t batch, int32_t task, tr_csx_t *C_out);
 int tr_batch_stats(tr_batch_t batch, int64_t *flops, int64_t *nnz_products, int64_t *nnz_out);
 int tr_batch_destroy(tr_batch_t batch);
+
+/* ---- serving loop ------------------------------------------------------------ */
+
+/* Continuous-batching request loop on one GPU.  Operands are registered once
+ * and stay resident (borrowed device CSR; a Galerkin operator P also keeps
+ * an on-device P^T, built on first use).  Requests:
+ *   TR_REQ_SPGEMM   C = op_a * op_b
+ *   TR_REQ_GALERKIN C = P^T (A P) with A = op_a, P = op_b (two product
+ *                   steps; AP stays resident between them).
+ * `workers` threads, each with its own stream, admit waiting steps in
+ * arrival order while their symbolic memory bound (tr_symbolic) fits
+ * `mem_budget`, and run them as one batch of up to `max_batch` steps;
+ * batches of different workers overlap on the device.  Results are
+ * bit-identical to the same product run alone.  No counterpart in the
+ * reference, which has no serving layer. */
+#define TR_REQ_SPGEMM 0
+#define TR_REQ_GALERKIN 1
+#define TR_REQ_DISCARD 0x100  /* or-ed into the kind: drop the result when done */
+typedef struct tr_server *tr_server_t;
+int tr_server_create(int device, size_t mem_budget, int32_t workers, int32_t max_batch, tr_server_t *server);
+int tr_server_register(tr_server_t server, const tr_csx_t *M, int32_t *op_id);
+int tr_server_submit(tr_server_t server, int32_t kind_flags, int32_t op_a, int32_t op_b, int64_t *req_id);
+/* Block until the request finished; C (optional) receives a view of the
+ * result, valid until tr_server_release. */
+int tr_server_wait(tr_server_t server, int64_t req_id, tr_csx_t *C, double *latency_ms);
+int tr_server_release(tr_server_t server, int64_t req_id);
+/* Completed requests, batches run, latency percentiles since the last
+ * reset, and throughput (2 * multiplications / wall time). */
+int tr_server_stats(tr_server_t server, int64_t *completed, int64_t *batches, double *p50_ms, double *p99_ms,
+                    double *gflops);
+int tr_server_reset_stats(tr_server_t server);
+int tr_server_destroy(tr_server_t server);
 
 /* ---- bitmask tile compression (SpaComm) --------------------------------- */
 

@@ -70,6 +70,15 @@ SIGNATURES = [
     ("tr_batch_result", ctypes.c_int, [_vp, _i32, _csx]),
     ("tr_batch_stats", ctypes.c_int, [_vp, _P(_i64), _P(_i64), _P(_i64)]),
     ("tr_batch_destroy", ctypes.c_int, [_vp]),
+    ("tr_server_create", ctypes.c_int, [ctypes.c_int, _sz, _i32, _i32, _P(_vp)]),
+    ("tr_server_register", ctypes.c_int, [_vp, _csx, _P(_i32)]),
+    ("tr_server_submit", ctypes.c_int, [_vp, _i32, _i32, _i32, _P(_i64)]),
+    ("tr_server_wait", ctypes.c_int, [_vp, _i64, _csx, _P(ctypes.c_double)]),
+    ("tr_server_release", ctypes.c_int, [_vp, _i64]),
+    ("tr_server_stats", ctypes.c_int, [_vp, _P(_i64), _P(_i64), _P(ctypes.c_double), _P(ctypes.c_double),
+                                       _P(ctypes.c_double)]),
+    ("tr_server_reset_stats", ctypes.c_int, [_vp]),
+    ("tr_server_destroy", ctypes.c_int, [_vp]),
     ("tr_bitmask_gen", ctypes.c_int, [_vp, _vp, _i32, _vp]),
     ("tr_bitmask_and", ctypes.c_int, [_vp, _vp, _vp, _i32, _vp]),
     ("tr_compress", ctypes.c_int, [_vp, _csx, _vp, _csx]),

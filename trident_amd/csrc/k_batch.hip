@@ -321,13 +321,14 @@ static int bguard(tr_ctx *ctx, F &&body) {
 extern "C" {
 
 int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_csx_t *B, const tr_csx_t *C_in,
-                    int32_t compress_mode, tr_batch_t *out) {
+                    int32_t flags, tr_batch_t *out) {
     if (!ctx || !out) return TR_ERR_INVALID;
     *out = nullptr;
     tr_batch *b = nullptr;
     const int rc = bguard(ctx, [&] {
         TR_REQUIRE(ntasks >= 1 && A && B, "tr_batch_create: need at least one task");
-        TR_REQUIRE(compress_mode == 0 || compress_mode == 1, "tr_batch_create: compress_mode must be 0 or 1");
+        TR_REQUIRE((flags & ~(TR_BATCH_COMPRESS | TR_BATCH_NO_GRAPH)) == 0, "tr_batch_create: unknown flags");
+        const int32_t compress_mode = (flags & TR_BATCH_COMPRESS) ? 1 : 0;
         b = new tr_batch();
         b->ctx = ctx;
         b->nt = ntasks;
@@ -417,6 +418,11 @@ int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_cs
             b->o_nnz[t] = (t + 1 < ntasks ? b->o_base[t + 1] : b->o_total) - b->o_base[t];
         for (int t = 0; t < ntasks; ++t)
             if (b->o_nnz[t] > INT32_MAX) fail(TR_ERR_OVERFLOW, "batch: an accumulated tile exceeds INT32_MAX entries");
+        b->ran = true;  // the eager pass already produced the results
+        if (flags & TR_BATCH_NO_GRAPH) {
+            ctx->timing = timing;
+            return;
+        }
         TR_HIP(hipStreamSynchronize(ctx->stream));
         // capture the same sequence with fixed sizes
         TR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -432,7 +438,6 @@ int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_cs
         TR_HIP(hipStreamEndCapture(ctx->stream, &b->graph));
         TR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         ctx->timing = timing;
-        b->ran = true;  // the eager pass already produced the results
     });
     if (rc != TR_OK) {
         delete b;
@@ -446,6 +451,7 @@ int tr_batch_run(tr_batch_t b) {
     if (!b) return TR_ERR_INVALID;
     tr_ctx *ctx = b->ctx;
     return bguard(ctx, [&] {
+        TR_REQUIRE(b->exec != nullptr, "tr_batch_run: batch was created with TR_BATCH_NO_GRAPH");
         ::tr::TimerScope ts(ctx, "batch_graph");
         TR_HIP(hipGraphLaunch(b->exec, ctx->stream));
         b->ran = true;

@@ -303,7 +303,7 @@ class Batch:
     (tr_batch_create / tr_batch_run): C_out_t = C_in_t + A_t * B_t."""
 
     def __init__(self, ctx: Context, A: List[DeviceCsx], B: List[DeviceCsx],
-                 C_in: Optional[List[Optional[DeviceCsx]]] = None, compress: bool = False):
+                 C_in: Optional[List[Optional[DeviceCsx]]] = None, compress: bool = False, graph: bool = True):
         n = len(A)
         assert len(B) == n and (C_in is None or len(C_in) == n)
         self.ctx = ctx
@@ -314,8 +314,8 @@ class Batch:
         if C_in is not None:
             arrC = (tr_csx_t * n)(*[(c.d if c is not None else tr_csx_t()) for c in C_in])
         h = ctypes.c_void_p()
-        ctx.check(ctx.lib.tr_batch_create(ctx.h, n, arrA, arrB, arrC, 1 if compress else 0, ctypes.byref(h)),
-                  "tr_batch_create")
+        flags = (1 if compress else 0) | (0 if graph else 2)
+        ctx.check(ctx.lib.tr_batch_create(ctx.h, n, arrA, arrB, arrC, flags, ctypes.byref(h)), "tr_batch_create")
         self.h = h
         self._keep = (A, B, C_in)  # operands must outlive the batch
 
@@ -335,6 +335,73 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             self.ctx.lib.tr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+TR_REQ_SPGEMM = 0
+TR_REQ_GALERKIN = 1
+TR_REQ_DISCARD = 0x100
+
+
+class Server:
+    """Continuous-batching request loop (tr_server_*): SpGEMM and Galerkin
+    requests on resident operands, admitted by their symbolic memory bound."""
+
+    def __init__(self, mem_budget: int, workers: int = 2, max_batch: int = 16, device: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.tr_server_create(device, mem_budget, workers, max_batch, ctypes.byref(h))
+        if rc != _lib.TR_OK:
+            raise TridentError(rc, "tr_server_create failed")
+        self.h = h
+        self._ops = []
+
+    def register(self, m: DeviceCsx) -> int:
+        op = ctypes.c_int32()
+        rc = self.lib.tr_server_register(self.h, ctypes.byref(m.d), ctypes.byref(op))
+        if rc != _lib.TR_OK:
+            raise TridentError(rc, "tr_server_register failed")
+        self._ops.append(m)
+        return int(op.value)
+
+    def submit(self, kind: int, a: int, b: int, discard: bool = False) -> int:
+        rid = ctypes.c_int64()
+        rc = self.lib.tr_server_submit(self.h, kind | (TR_REQ_DISCARD if discard else 0), a, b, ctypes.byref(rid))
+        if rc != _lib.TR_OK:
+            raise TridentError(rc, "tr_server_submit failed")
+        return int(rid.value)
+
+    def wait(self, rid: int, want_result: bool = True):
+        d = tr_csx_t()
+        lat = ctypes.c_double()
+        rc = self.lib.tr_server_wait(self.h, rid, ctypes.byref(d) if want_result else None, ctypes.byref(lat))
+        if rc != _lib.TR_OK:
+            raise TridentError(rc, f"request {rid} failed")
+        return (d if want_result else None), float(lat.value)
+
+    def release(self, rid: int):
+        self.lib.tr_server_release(self.h, rid)
+
+    def stats(self) -> dict:
+        c, b = ctypes.c_int64(), ctypes.c_int64()
+        p50, p99, gf = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self.lib.tr_server_stats(self.h, ctypes.byref(c), ctypes.byref(b), ctypes.byref(p50), ctypes.byref(p99),
+                                 ctypes.byref(gf))
+        return {"completed": c.value, "batches": b.value, "p50_ms": p50.value, "p99_ms": p99.value,
+                "GFLOP/s": gf.value}
+
+    def reset_stats(self):
+        self.lib.tr_server_reset_stats(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tr_server_destroy(self.h)
             self.h = None
 
     def __del__(self):
