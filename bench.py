@@ -204,6 +204,24 @@ def stage_bench(args):
         res[mode] = {"ms": round(1000 * t_med, 2), "GFLOP/s": round(2 * mults / t_med / 1e9, 2)}
         log(f"[stage] {mode}: {res[mode]}")
     res["speedup_batched_vs_plain"] = round(res["plain"]["ms"] / res["batched"]["ms"], 3)
+    # per-kernel HIP-event times: one plain pass and one eager (uncaptured) batch pass
+    for mode in ("plain", "batched_eager"):
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        if mode == "plain":
+            for C in plain_step():
+                C.free()
+        else:
+            b = Batch(ctx, [dT[(i, k)] for i, j in keys], [dT[(k, j)] for i, j in keys], Cin, compress=False,
+                      graph=False)
+            ctx.synchronize()
+            ctx.reset_timing()
+            b.run()
+            ctx.synchronize()
+            b.close()
+        ctx.set_timing(False)
+        res[mode + "_kernels_ms"] = {kk: round(v[1], 3) for kk, v in sorted(ctx.timings().items())}
+        log(f"[stage] {mode} kernels: {res[mode + '_kernels_ms']}")
     out = {
         "metric": METRIC, "value": res["batched"]["GFLOP/s"], "unit": "GFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": 1, "ms_per_step": res["batched"]["ms"], "higher_is_better": True,
@@ -387,6 +405,33 @@ def serve_bench(args):
     return 0
 
 
+def copy_bench(args):
+    """Plain device copy of a known size (an elementwise y = x * 1 over fp64):
+    the calibration run for the FETCH_SIZE / WRITE_SIZE counters, whose
+    readings for this kernel are compared with the bytes it must move."""
+    import torch
+    gib = float(args.workload.split(":")[1])
+    n = int(gib * (1 << 30)) // 8
+    x = torch.rand(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(max(args.warmup, 1)):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / max(args.steps, 1)
+    nbytes = 8 * n
+    out = {"metric": "device copy GB/s", "value": round(2 * nbytes / dt / 1e9, 2), "unit": "GB/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000 * dt, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"copy of {nbytes} bytes (read {nbytes}, write {nbytes} per launch)",
+                      "read_KiB": nbytes / 1024, "write_KiB": nbytes / 1024}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -410,6 +455,8 @@ def main():
         return galerkin_bench(args)
     if args.workload.startswith("serve"):
         return serve_bench(args)
+    if args.workload.startswith("copy:"):
+        return copy_bench(args)
 
     from trident_amd.device import Context, spgemm_panels
 
