@@ -137,8 +137,9 @@ int tr_spgemm_flops(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t 
  *  - flags: TR_BATCH_COMPRESS applies bitmask compression with
  *    mask = non-empty columns of A[t] AND non-empty rows of B[t] (the filter
  *    of SpaComm, include/spacomm.cuh:165-352, :962-1091); TR_BATCH_NO_GRAPH
- *    runs the batch once without capturing it (one-shot batches, e.g. the
- *    serving loop).
+ *    skips the capture (one-shot batches, e.g. the serving loop): then
+ *    tr_batch_run enqueues the same fixed-size launch sequence eagerly,
+ *    kernel by kernel, which also lets per-kernel timing see it.
  * tr_batch_create plans the stage (one eager pass that sizes every buffer
  * and already produces the results) and captures it; tr_batch_run replays
  * the graph on the context stream with no host round trip.  Inputs must keep

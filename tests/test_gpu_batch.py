@@ -92,3 +92,24 @@ def test_batch_equals_sequential_sp_mma(ctx):
         H, R = b.result(t).download(), Cs[t].download()
         assert same(H, R)
     b.close()
+
+
+@pytest.mark.parametrize("compress", [False, True])
+def test_eager_batch_replays_match_graph(ctx, compress):
+    # a batch created without a graph re-runs the same launch sequence
+    # eagerly; its results equal the captured batch's on every run
+    A = gen.rmat(11, 16, seed=9)
+    T = tiles_of(A, 4)
+    dT = {key: ctx.upload(m) for key, m in T.items()}
+    keys = [(i, j) for i in range(4) for j in range(4)]
+    As, Bs = [dT[(i, 1)] for i, j in keys], [dT[(1, j)] for i, j in keys]
+    bg = Batch(ctx, As, Bs, None, compress=compress)
+    be = Batch(ctx, As, Bs, None, compress=compress, graph=False)
+    for rep in range(2):
+        bg.run()
+        be.run()
+        ctx.synchronize()
+        for t in range(len(keys)):
+            assert same(be.result(t).download(), bg.result(t).download()), f"run {rep} task {keys[t]}"
+    bg.close()
+    be.close()

@@ -451,9 +451,12 @@ int tr_batch_run(tr_batch_t b) {
     if (!b) return TR_ERR_INVALID;
     tr_ctx *ctx = b->ctx;
     return bguard(ctx, [&] {
-        TR_REQUIRE(b->exec != nullptr, "tr_batch_run: batch was created with TR_BATCH_NO_GRAPH");
-        ::tr::TimerScope ts(ctx, "batch_graph");
-        TR_HIP(hipGraphLaunch(b->exec, ctx->stream));
+        if (b->exec) {
+            ::tr::TimerScope ts(ctx, "batch_graph");
+            TR_HIP(hipGraphLaunch(b->exec, ctx->stream));
+        } else {
+            batch_body(b, true);  // the sequence the graph would hold, launched eagerly
+        }
         b->ran = true;
     });
 }

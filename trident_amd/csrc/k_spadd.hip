@@ -6,12 +6,14 @@
 // 799-905: csrgeam2 bufferSize / Nnz / compute with a host sync after each).
 //
 // Merge-path formulation: every row's union is cut into chunks of kChunk
-// merged positions; a wave owns a chunk and finds, per lane, the merge-path
-// split of its position by binary search (ties take X's entry first, so an
-// equal column appears as an adjacent (X, Y) pair that collapses into one
-// output entry x + y).  Pass 1 counts the kept positions per chunk, one scan
-// over all chunks gives both the row pointers and every chunk's output
-// offset, pass 2 writes.  Long rows are spread over many waves.
+// merged positions; a wave owns a chunk, splits its two ends by binary
+// search in global memory (ties take X's entry first, so an equal column
+// appears as an adjacent (X, Y) pair that collapses into one output entry
+// x + y), stages the chunk's X and Y ranges in LDS and merges them there,
+// kChunk / 64 positions per lane.  Pass 1 counts the kept positions per
+// chunk, one scan over all chunks gives both the row pointers and every
+// chunk's output offset, pass 2 writes.  Long rows are spread over many
+// waves.
 //
 // Like the multiply, the add runs over a table of tasks (a plain call is a
 // table of one; the batched stage executor adds all tile products into
@@ -73,12 +75,10 @@ __global__ void k_add_chunk_rows(const int64_t *__restrict__ cstart, int32_t row
     for (int64_t c = cstart[g]; c < cstart[g + 1]; ++c) crow[c] = g;
 }
 
-// merged element at position t of a row (X_i, Y_i); returns key, value and
-// whether the position is kept (a Y entry equal to the preceding X entry is
-// folded into it)
-__device__ __forceinline__ bool merge_at(const int32_t *__restrict__ xi, const double *__restrict__ xv, int32_t lx,
-                                         const int32_t *__restrict__ yi, const double *__restrict__ yv, int32_t ly,
-                                         int32_t t, int32_t &key, double &val, bool want_val) {
+// merge-path split of diagonal t: number of X entries among the first t
+// merged positions (ties take X first)
+__device__ __forceinline__ int32_t merge_split(const int32_t *__restrict__ xi, int32_t lx,
+                                               const int32_t *__restrict__ yi, int32_t ly, int32_t t) {
     int32_t lo = t - ly > 0 ? t - ly : 0;
     int32_t hi = t < lx ? t : lx;
     while (lo < hi) {
@@ -86,33 +86,36 @@ __device__ __forceinline__ bool merge_at(const int32_t *__restrict__ xi, const d
         if (xi[mid] <= yi[t - 1 - mid]) lo = mid + 1;
         else hi = mid;
     }
-    const int32_t i = lo, j = t - lo;
-    const bool fromX = (i < lx) && (j >= ly || xi[i] <= yi[j]);
-    if (fromX) {
-        key = xi[i];
-        if (want_val) {
-            val = xv[i];
-            if (j < ly && yi[j] == key) val = val + yv[j];
-        }
-        return true;
-    }
-    key = yi[j];
-    if (i > 0 && xi[i - 1] == key) return false;  // folded into the X entry
-    if (want_val) val = yv[j];
-    return true;
+    return lo;
 }
 
-// chunk count is read on the device (cstart[rows]); grid-stride so the
-// launch can be replayed from a graph
+constexpr int kPerLane = kChunk / kWave;  // merged positions per lane
+constexpr int32_t kGuardLo = INT32_MIN;   // "no X entry before the chunk"
+constexpr int32_t kGuardHi = INT32_MAX;   // "no Y entry after the chunk"
+
+// One wave per chunk of kChunk merged positions of a row.  The chunk's ends
+// are split once in global memory; its X and Y ranges (plus the X entry
+// before it and the Y entry after it, which decide folding at the edges) are
+// staged in LDS with coalesced loads, and each lane merges kPerLane
+// consecutive positions from LDS.  A Y entry equal to the preceding X entry
+// is folded into it (output x + y at the X position).  The chunk count is
+// read on the device (cstart[rows]); grid-stride so the launch can be
+// replayed from a graph.
 template <bool kWrite>
 __global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__ tasks, int nt, int32_t rows,
                                                   const int64_t *__restrict__ cstart,
                                                   const int32_t *__restrict__ crow, int32_t *__restrict__ ccount,
                                                   const int64_t *__restrict__ coff, int32_t *__restrict__ oi,
                                                   double *__restrict__ ov) {
-    const int l = lane_id();
+    constexpr int kVW = kWrite ? 4 : 1;
+    constexpr int kVN = kWrite ? kChunk + 2 : 1;
+    __shared__ int32_t s_xi[4][kChunk + 2], s_yi[4][kChunk + 2];
+    __shared__ double s_xv[kVW][kVN], s_yv[kVW][kVN];
+    const int l = lane_id(), w = threadIdx.x / kWave;
+    int32_t *XI = s_xi[w], *YI = s_yi[w];
+    double *XV = s_xv[kWrite ? w : 0], *YV = s_yv[kWrite ? w : 0];
     const int64_t nchunks = cstart[rows];
-    for (int64_t c = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave; c < nchunks; c += (int64_t)gridDim.x * 4) {
+    for (int64_t c = (int64_t)blockIdx.x * 4 + w; c < nchunks; c += (int64_t)gridDim.x * 4) {
         const int32_t g = crow[c];
         const AddTaskDev &T = tasks[add_task_of(tasks, nt, g)];
         const RowSpan s = row_span(T, g);
@@ -121,25 +124,79 @@ __global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__
         const int32_t *__restrict__ yi = T.yi + s.ys;
         const double *__restrict__ yv = T.yv + s.ys;
         const int32_t t0 = (int32_t)(c - cstart[g]) * kChunk;
-        const int32_t tend = min(t0 + kChunk, s.lx + s.ly);
-        int64_t out = kWrite ? coff[c] : 0;
-        int32_t kept_total = 0;
-        for (int32_t tb = t0; tb < tend; tb += kWave) {
-            const int32_t t = tb + l;
-            int32_t key = 0;
-            double val = 0.0;
-            bool keep = false;
-            if (t < tend) keep = merge_at(xi, xv, s.lx, yi, yv, s.ly, t, key, val, kWrite);
-            const uint64_t msk = ballot(keep);
-            if (kWrite && keep) {
-                const int64_t q = out + rank_below(msk);
-                oi[q] = key;
-                ov[q] = val;
-            }
-            out += __popcll(msk);
-            kept_total += __popcll(msk);
+        const int32_t t1 = min(t0 + kChunk, s.lx + s.ly);
+        // lanes 0 and 1 split the two ends of the chunk
+        int32_t sp = 0;
+        if (l < 2) sp = merge_split(xi, s.lx, yi, s.ly, l == 0 ? t0 : t1);
+        const int32_t i0 = __shfl(sp, 0, kWave), i1 = __shfl(sp, 1, kWave);
+        const int32_t j0 = t0 - i0, j1 = t1 - i1;
+        const int32_t nx = i1 - i0, ny = j1 - j0;
+        // XI[u] = X[i0 - 1 + u] for u in [0, nx + 1); YI[u] = Y[j0 + u] for u in [0, ny + 1)
+        for (int u = l; u <= nx; u += kWave) {
+            const int32_t ii = i0 - 1 + u;
+            XI[u] = ii >= 0 ? xi[ii] : kGuardLo;
+            if (kWrite) XV[u] = ii >= 0 ? xv[ii] : 0.0;
         }
-        if (!kWrite && l == 0) ccount[c] = kept_total;
+        for (int u = l; u <= ny; u += kWave) {
+            const int32_t jj = j0 + u;
+            YI[u] = jj < s.ly ? yi[jj] : kGuardHi;
+            if (kWrite) YV[u] = jj < s.ly ? yv[jj] : 0.0;
+        }
+        wave_sync();
+        // this lane's first local position and its split inside the chunk
+        const int32_t d = l * kPerLane, n = nx + ny;
+        int32_t i = 0, j = 0;
+        if (d < n) {
+            int32_t lo = d - ny > 0 ? d - ny : 0, hi = d < nx ? d : nx;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (XI[1 + mid] <= YI[d - 1 - mid]) lo = mid + 1;
+                else hi = mid;
+            }
+            i = lo;
+            j = d - lo;
+        }
+        int32_t keys[kPerLane];
+        double vals[kPerLane];
+        uint32_t keep = 0;
+#pragma unroll
+        for (int u = 0; u < kPerLane; ++u) {
+            keys[u] = 0;
+            vals[u] = 0.0;
+            if (d + u >= n) continue;
+            const bool fromX = (i < nx) && (j >= ny || XI[1 + i] <= YI[j]);
+            if (fromX) {
+                keys[u] = XI[1 + i];
+                if (kWrite) {
+                    vals[u] = XV[1 + i];
+                    if (YI[j] == keys[u]) vals[u] = vals[u] + YV[j];
+                }
+                keep |= 1u << u;
+                ++i;
+            } else {
+                keys[u] = YI[j];
+                if (XI[i] != keys[u]) {  // else folded into the preceding X entry
+                    if (kWrite) vals[u] = YV[j];
+                    keep |= 1u << u;
+                }
+                ++j;
+            }
+        }
+        const int nkeep = __popc(keep);
+        const int32_t incl = wave_incl_scan((int32_t)nkeep);
+        if (kWrite) {
+            int64_t q = coff[c] + (incl - nkeep);
+#pragma unroll
+            for (int u = 0; u < kPerLane; ++u)
+                if (keep & (1u << u)) {
+                    oi[q] = keys[u];
+                    ov[q] = vals[u];
+                    ++q;
+                }
+        } else if (l == kWave - 1) {
+            ccount[c] = incl;
+        }
+        wave_sync();  // XI/YI are rewritten for the next chunk
     }
 }
 

@@ -77,17 +77,6 @@ __device__ __forceinline__ int32_t gallop_dev(const int32_t *__restrict__ a, int
     return hi;
 }
 
-__device__ __forceinline__ int owner_lane32(int32_t excl, int32_t x) {
-    int lo = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-        int probe = lo + step;
-        int32_t v = __shfl(excl, probe & 63, kWave);
-        if (probe < kWave && v <= x) lo = probe;
-    }
-    return lo;
-}
-
 // Visit the column indices of B row segment [s, e) with the wave's lanes,
 // four loads in flight per lane before the first is consumed.
 template <typename F>
@@ -547,118 +536,77 @@ __global__ __launch_bounds__(256) void k_num_light(const TaskDev *__restrict__ t
     }
 }
 
-template <int T>
-struct HeavyLds {
-    int32_t keys[T];
+// ---------------------------------------------------------------- numeric, heavy rows (workgroup)
+// One NT-thread workgroup per item (<= kItemWindows windows of one row),
+// sharing one T-slot LDS table per window.  For each window the row's A
+// entries are taken NT at a time (one per thread): each thread finds the
+// part of B_k inside the window (register cursor for rows with <= NT
+// entries, the item's cursor array otherwise), a workgroup scan flattens the
+// products in A-order, and each step applies NT of them.  Slot keys carry the
+// column offset in the high half and an ordering tag in the low half: two
+// products of one step that meet in a slot are applied in thread order (=
+// A-order) through atomicMin rounds.  A presence bitmap over the window gives
+// every entry its sorted rank (prefix popcounts) for the write-out.
+constexpr uint32_t kSlotFree = 0xffffffffu;
+constexpr uint32_t kNoTag = 0xffffu;
+
+template <int T, int NT>
+struct HeavyWgLds {
+    uint32_t keytag[T];
     double vals[T];
-    int32_t tags[T];
     unsigned long long bm[kBmWords];
-    int32_t wpre[kBmWords];
+    uint16_t wpre[kBmWords];
+    int32_t seg_s[NT];
+    int32_t seg_x[NT + 1];
+    double seg_a[NT];
+    int32_t wsum[NT / kWave];
+    int32_t flag[2];
+    int32_t nocc;
 };
 
-// Insert and apply the products of one batch: lane l holds A entry value a
-// and the B range [s, e) of its column; products are flattened in lane order.
-template <int T>
-__device__ __forceinline__ void heavy_batch(HeavyLds<T> &L, const int32_t *__restrict__ bi,
-                                            const double *__restrict__ bv, int32_t s, int32_t e, double a, int32_t c0,
-                                            int &nocc, Counters *__restrict__ cnt) {
-    constexpr uint32_t kLog = __builtin_ctz(T);
-    const int l = lane_id();
-    const int32_t len = e - s;
-    const int32_t incl = wave_incl_scan(len);
-    const int32_t excl = incl - len;
-    const int32_t total = __shfl(incl, kWave - 1, kWave);
-    if (total == 0) return;
-    // software pipeline: the B entries of step x0 + 64 are loaded while
-    // step x0 is inserted and applied
-    int32_t jn = 0;
-    double bn = 0.0, an = 0.0;
-    {
-        const int o = owner_lane32(excl, l);
-        const int32_t so = __shfl(s, o, kWave), eo = __shfl(excl, o, kWave);
-        an = __shfl(a, o, kWave);
-        if (l < total) {
-            const int32_t q = so + (l - eo);
-            jn = bi[q];
-            bn = bv[q];
-        }
+// workgroup exclusive scan of v; returns the exclusive prefix, *total = sum
+template <int NT>
+__device__ __forceinline__ int32_t wg_excl_scan(int32_t v, int32_t *wsum, int32_t *total) {
+    const int w = threadIdx.x / kWave, l = lane_id();
+    const int32_t inc = wave_incl_scan(v);
+    if (l == kWave - 1) wsum[w] = inc;
+    __syncthreads();
+    int32_t off = 0, tot = 0;
+#pragma unroll
+    for (int u = 0; u < NT / kWave; ++u) {
+        const int32_t s = wsum[u];
+        off += (u < w) ? s : 0;
+        tot += s;
     }
-    for (int32_t x0 = 0; x0 < total; x0 += kWave) {
-        const int32_t x = x0 + l;
-        bool act = x < total;
-        const int32_t j = jn;
-        const double b = bn, ao = an;
-        if (x0 + kWave < total) {
-            const int32_t xn = x + kWave;
-            const int o = owner_lane32(excl, xn);
-            const int32_t so = __shfl(s, o, kWave), eo = __shfl(excl, o, kWave);
-            an = __shfl(a, o, kWave);
-            if (xn < total) {
-                const int32_t q = so + (xn - eo);
-                jn = bi[q];
-                bn = bv[q];
-            }
-        }
-        uint32_t h = 0;
-        bool fresh = false;
-        if (act) {
-            h = hash_slot(j, kLog);
-            bool placed = false;
-            for (int probe = 0; probe < T; ++probe) {
-                const int32_t old = atomicCAS(&L.keys[h], kEmpty, j);
-                if (old == kEmpty) {
-                    L.vals[h] = 0.0;
-                    const int32_t bpos = j - c0;
-                    atomicOr(&L.bm[bpos >> 6], 1ull << (bpos & 63));
-                    placed = fresh = true;
-                    break;
-                }
-                if (old == j) {
-                    placed = true;
-                    break;
-                }
-                h = (h + 1) & (T - 1);
-            }
-            if (!placed) {
-                // table full: only reachable if symbolic counts are wrong
-                atomicOr(&cnt->overflow, 8);
-                act = false;
-            }
-        }
-        // count newly occupied slots (wave-local counter)
-        nocc += __popcll(ballot(fresh));
-        wave_sync();
-        bool pending = act;
-        while (ballot(pending)) {
-            if (pending) atomicMin((uint32_t *)&L.tags[h], (uint32_t)l);
-            wave_sync();
-            const bool mine = pending && L.tags[h] == l;
-            if (mine) L.vals[h] = __fma_rn(ao, b, L.vals[h]);
-            wave_sync();
-            if (mine) {
-                L.tags[h] = -1;
-                pending = false;
-            }
-            wave_sync();
-        }
-    }
+    *total = tot;
+    return off + inc - v;
 }
 
-template <int T>
-__global__ __launch_bounds__(kWave) void k_num_heavy(const TaskDev *__restrict__ tasks, int nt,
+// largest o in [0, n) with x[o] <= key (x nondecreasing, x[0] == 0 <= key)
+__device__ __forceinline__ int owner_lds(const int32_t *x, int n, int32_t key) {
+    int lo = 0, hi = n;  // x[lo] <= key < x[hi] (x[n] treated as +inf)
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] <= key) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int T, int NT>
+__global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__ tasks, int nt,
                                                      const Item *__restrict__ items, Counters *__restrict__ cnt,
                                                      const Window *__restrict__ wins,
                                                      const int64_t *__restrict__ cursor_base,
                                                      int32_t *__restrict__ cursors, const int64_t *__restrict__ rowptr,
                                                      const Panel *__restrict__ panels, int npanels) {
-    __shared__ HeavyLds<T> L;
-    const int l = lane_id();
-    for (int t = l; t < T; t += kWave) {
-        L.keys[t] = kEmpty;
-        L.tags[t] = -1;
-    }
-    for (int t = l; t < kBmWords; t += kWave) L.bm[t] = 0ull;
-    wave_sync();
+    constexpr uint32_t kLog = __builtin_ctz(T);
+    __shared__ HeavyWgLds<T, NT> L;
+    const int tid = threadIdx.x, l = lane_id();
+    for (int t = tid; t < T; t += NT) L.keytag[t] = kSlotFree;
+    for (int t = tid; t < kBmWords; t += NT) L.bm[t] = 0ull;
+    if (tid == 0) L.nocc = 0;
+    __syncthreads();
     const int32_t nitems = cnt->items;
     for (int32_t itx = blockIdx.x; itx < nitems; itx += gridDim.x) {
         const Item I = items[itx];
@@ -677,92 +625,171 @@ __global__ __launch_bounds__(kWave) void k_num_heavy(const TaskDev *__restrict__
         const Panel &P = find_panel(panels, npanels, g);
         int32_t *oi_row = P.idx + (rs - P.nnz_begin);
         double *ov_row = P.val + (rs - P.nnz_begin);
-        const bool short_row = (a1 - a0) <= kWave;
-        // short rows: the A entry and B range of lane l live in registers
+        const bool short_row = (a1 - a0) <= NT;
+        // short rows: the A entry and B range of thread tid live in registers
         int32_t rk_bs = 0, rk_be = 0, rk_cur = 0;
         double rk_a = 0.0;
-        if (short_row && a0 + l < a1) {
-            const int32_t k = ai[a0 + l];
-            rk_a = av[a0 + l];
+        if (short_row && a0 + tid < a1) {
+            const int32_t k = ai[a0 + tid];
+            rk_a = av[a0 + tid];
             rk_bs = bp[k];
             rk_be = bp[k + 1];
         }
         bool have_cursor = false;
         for (int32_t wi = I.wbeg; wi < I.wend; ++wi) {
             const Window W = wins[wi];
-            if (W.cnt == 0) continue;  // no B entry of this row's A-columns lies in it
+            if (W.cnt == 0) continue;  // block-uniform
             const bool full = (W.c0 == 0 && W.c1 >= ncols);
-            int nocc = 0;
-            if (short_row) {
-                int32_t s = rk_bs, e = rk_be;
-                if (!full && a0 + l < a1) {
-                    s = have_cursor ? rk_cur : lower_bound_dev(bi, rk_bs, rk_be, W.c0);
-                    e = gallop_dev(bi, s, rk_be, W.c1);
-                    rk_cur = e;
-                }
-                heavy_batch<T>(L, bi, bv, s, e, rk_a, W.c0, nocc, cnt);
-            } else {
-                const bool last = (wi == I.wend - 1);
-                for (int32_t pb = a0; pb < a1; pb += kWave) {
-                    const int32_t p = pb + l;
-                    int32_t s = 0, e = 0;
-                    double a = 0.0;
-                    if (p < a1) {
+            const bool last = (wi == I.wend - 1);
+            int nfresh = 0;
+            for (int32_t pb = a0; pb < a1; pb += NT) {
+                // this thread's k-slot: A entry p, B segment [s, e) inside the window
+                const int32_t p = pb + tid;
+                int32_t s = 0, e = 0;
+                double a = 0.0;
+                if (p < a1) {
+                    int32_t bs, be;
+                    if (short_row) {
+                        bs = rk_bs;
+                        be = rk_be;
+                        a = rk_a;
+                    } else {
                         const int32_t k = ai[p];
                         a = av[p];
-                        const int32_t bs = bp[k], be = bp[k + 1];
-                        if (full) {
-                            s = bs;
-                            e = be;
-                        } else {
-                            s = have_cursor ? cur[p - a0] : lower_bound_dev(bi, bs, be, W.c0);
-                            e = gallop_dev(bi, s, be, W.c1);
-                            if (!last) cur[p - a0] = e;
+                        bs = bp[k];
+                        be = bp[k + 1];
+                    }
+                    if (full) {
+                        s = bs;
+                        e = be;
+                    } else {
+                        const int32_t c = short_row ? rk_cur : (have_cursor ? cur[p - a0] : 0);
+                        s = have_cursor ? c : lower_bound_dev(bi, bs, be, W.c0);
+                        e = gallop_dev(bi, s, be, W.c1);
+                        if (short_row) rk_cur = e;
+                        else if (!last) cur[p - a0] = e;
+                    }
+                }
+                int32_t total;
+                const int32_t excl = wg_excl_scan<NT>(e - s, L.wsum, &total);
+                L.seg_s[tid] = s;
+                L.seg_x[tid] = excl;
+                L.seg_a[tid] = a;
+                const int nk = min(NT, a1 - pb);
+                __syncthreads();
+                // products x = x0 + tid, flattened in A-order; the B entry of the
+                // next step is loaded while the current step is applied
+                int32_t jn = 0;
+                double bn = 0.0, an = 0.0;
+                if (tid < total) {
+                    const int o = owner_lds(L.seg_x, nk, tid);
+                    const int32_t q = L.seg_s[o] + (tid - L.seg_x[o]);
+                    an = L.seg_a[o];
+                    jn = bi[q];
+                    bn = bv[q];
+                }
+                for (int32_t x0 = 0; x0 < total; x0 += NT) {
+                    const int32_t x = x0 + tid;
+                    bool act = x < total;
+                    const int32_t j = jn;
+                    const double b = bn, ao = an;
+                    const int32_t xn = x + NT;
+                    if (xn < total) {
+                        const int o = owner_lds(L.seg_x, nk, xn);
+                        const int32_t q = L.seg_s[o] + (xn - L.seg_x[o]);
+                        an = L.seg_a[o];
+                        jn = bi[q];
+                        bn = bv[q];
+                    }
+                    uint32_t h = 0, key = 0;
+                    if (act) {
+                        const uint32_t off = (uint32_t)(j - W.c0);
+                        key = off << 16;
+                        h = hash_slot((int32_t)off, kLog);
+                        bool placed = false;
+                        for (int probe = 0; probe < T; ++probe) {
+                            const uint32_t old = atomicCAS(&L.keytag[h], kSlotFree, key | kNoTag);
+                            if (old == kSlotFree) {
+                                L.vals[h] = 0.0;
+                                atomicOr(&L.bm[off >> 6], 1ull << (off & 63));
+                                ++nfresh;
+                                placed = true;
+                                break;
+                            }
+                            if ((old >> 16) == off) {
+                                placed = true;
+                                break;
+                            }
+                            h = (h + 1) & (T - 1);
+                        }
+                        if (!placed) {
+                            atomicOr(&cnt->overflow, 8);  // table full: symbolic counts were wrong
+                            act = false;
                         }
                     }
-                    heavy_batch<T>(L, bi, bv, s, e, a, W.c0, nocc, cnt);
+                    if (tid == 0) L.flag[0] = 0;
+                    __syncthreads();
+                    // apply in thread order: the lowest pending thread of each slot per round
+                    bool pending = act;
+                    for (int round = 0;; ++round) {
+                        if (pending) atomicMin(&L.keytag[h], key | (uint32_t)tid);
+                        __syncthreads();
+                        if (pending && (L.keytag[h] & kNoTag) == (uint32_t)tid) {
+                            L.vals[h] = __fma_rn(ao, b, L.vals[h]);
+                            L.keytag[h] = key | kNoTag;
+                            pending = false;
+                        }
+                        if (ballot(pending) && l == 0) L.flag[round & 1] = 1;
+                        if (tid == 0) L.flag[(round + 1) & 1] = 0;
+                        __syncthreads();
+                        if (!L.flag[round & 1]) break;
+                    }
                 }
+                __syncthreads();  // seg_* are rewritten by the next batch
             }
             if (!full) have_cursor = true;
-            wave_sync();
-            // prefix popcounts of the presence words (8 words per lane)
+            // entries created in this window (each thread counted its fresh inserts)
+            const int32_t wf = wave_sum(nfresh);
+            if (l == 0 && wf) atomicAdd(&L.nocc, wf);
+            // prefix popcounts of the presence words
             const int nwords = (W.c1 - W.c0 + 63) >> 6;
-            int32_t wc[kBmWords / kWave];
+            constexpr int kPer = kBmWords / NT > 0 ? kBmWords / NT : 1;
+            int32_t wc[kPer];
             int32_t tsum = 0;
 #pragma unroll
-            for (int u = 0; u < kBmWords / kWave; ++u) {
-                const int wd = l * (kBmWords / kWave) + u;
-                wc[u] = wd < nwords ? __popcll(L.bm[wd]) : 0;
+            for (int u = 0; u < kPer; ++u) {
+                const int wd = tid * kPer + u;
+                wc[u] = (wd < nwords && wd < kBmWords) ? __popcll(L.bm[wd]) : 0;
                 tsum += wc[u];
             }
-            const int32_t tinc = wave_incl_scan(tsum);
-            {
-                int32_t run = tinc - tsum;
+            int32_t tot;
+            int32_t run = wg_excl_scan<NT>(tsum, L.wsum, &tot);
 #pragma unroll
-                for (int u = 0; u < kBmWords / kWave; ++u) {
-                    L.wpre[l * (kBmWords / kWave) + u] = run;
-                    run += wc[u];
-                }
+            for (int u = 0; u < kPer; ++u) {
+                const int wd = tid * kPer + u;
+                if (wd < kBmWords) L.wpre[wd] = (uint16_t)run;
+                run += wc[u];
             }
-            wave_sync();
-            if (l == 0 && nocc != W.cnt) atomicOr(&cnt->overflow, 4);
+            __syncthreads();
+            if (tid == 0 && L.nocc != W.cnt) atomicOr(&cnt->overflow, 4);
             int32_t *oi = oi_row + W.off;
             double *ov = ov_row + W.off;
-            for (int h = l; h < T; h += kWave) {
-                const int32_t k = L.keys[h];
-                if (k == kEmpty) continue;
-                const int32_t bpos = k - W.c0;
-                const int wd = bpos >> 6;
-                const int32_t rr = L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (bpos & 63)) - 1ull));
+            for (int h = tid; h < T; h += NT) {
+                const uint32_t kt = L.keytag[h];
+                if (kt == kSlotFree) continue;
+                const uint32_t off = kt >> 16;
+                const int wd = (int)(off >> 6);
+                const int32_t rr = (int32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
                 if (rr < W.cnt) {  // never write past the window
-                    oi[rr] = k;
+                    oi[rr] = W.c0 + (int32_t)off;
                     ov[rr] = L.vals[h];
                 }
             }
-            wave_sync();
-            for (int t = l; t < T; t += kWave) L.keys[t] = kEmpty;
-            for (int t = l; t < nwords; t += kWave) L.bm[t] = 0ull;
-            wave_sync();
+            __syncthreads();
+            for (int t = tid; t < T; t += NT) L.keytag[t] = kSlotFree;
+            for (int t = tid; t < nwords; t += NT) L.bm[t] = 0ull;
+            if (tid == 0) L.nocc = 0;
+            __syncthreads();
         }
     }
 }
@@ -908,8 +935,9 @@ void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *dp, int np) {
     if (pl.h.cls[4]) launch_num_light<1024>(ctx, pl, 4, dp, np);
     if (pl.h.cls[5]) launch_num_light<2048>(ctx, pl, 5, dp, np);
     if (pl.h.cls[kHeavyClass] && pl.h.items > 0) {
-        TR_LAUNCH(ctx, "num_heavy", k_num_heavy<kN2T>, cap_grid(pl.h.items, 1 << 24), kWave, 0, pl.d_tasks, pl.nt,
-                  pl.items, pl.cnt, pl.wins, pl.cbase, pl.cursors, pl.rowptr, dp, np);
+        TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<kN2T, kHeavyWgThreads>), cap_grid(pl.h.items, 1 << 24),
+                  kHeavyWgThreads, 0, pl.d_tasks, pl.nt, pl.items, pl.cnt, pl.wins, pl.cbase, pl.cursors, pl.rowptr,
+                  dp, np);
     }
 }
 

@@ -6,10 +6,10 @@
 #include <vector>
 
 #ifndef TR_HEAVY_T
-#define TR_HEAVY_T 1024
+#define TR_HEAVY_T 2048
 #endif
 #ifndef TR_WIN_H
-#define TR_WIN_H 384
+#define TR_WIN_H 896
 #endif
 
 namespace tr {
@@ -28,7 +28,8 @@ constexpr int kWinMaxWidth = 32768;                       // window width cap (c
 constexpr int kWidthShift = 15 - kFineShift;               // fine blocks per width quantum: 2^8
 constexpr int kBmWords = kWinMaxWidth / 64;                // presence words per window
 constexpr int kItemWindows = 16;   // windows per heavy work item
-constexpr int kHeavyThreads = 1024;
+constexpr int kHeavyThreads = 1024;  // symbolic heavy block
+constexpr int kHeavyWgThreads = 256;  // numeric heavy workgroup
 
 struct Window {
     int32_t c0, c1;   // column range [c0, c1)
