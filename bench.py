@@ -28,7 +28,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SpGEMM GFLOP/s + % HBM roofline; batched/served p50/p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
-LIGHT_MAX_FLOPS = 1024  # row class cut used by the library (flops <= cut: light kernels)
 
 
 def log(*a):
@@ -36,18 +35,33 @@ def log(*a):
 
 
 def build_workload(name):
-    from trident_amd import gen
+    from trident_amd import workloads
     if name == "er1024":
-        A = gen.erdos_renyi(1024, 8, seed=0)
         desc = "C = A*A, A = Erdos-Renyi n=1024, 8 uniform columns/row (seed 0), int32 CSR, fp64 U[-1,1)"
     elif name.startswith("rmat:"):
         s = int(name.split(":")[1])
-        A = gen.rmat(s, 16, seed=1)
         desc = (f"C = A*A, A = R-MAT scale {s} (edge factor 16, a,b,c,d=.57,.19,.19,.05, dups merged, seed 1), "
                 "int32 CSR, fp64 U[-1,1)")
     else:
         raise SystemExit(f"unknown workload {name}")
-    return A, desc
+    return workloads.product_operand(name), desc
+
+
+def apply_tuning(args):
+    """Tuning of the multiply for this run: the autotuner's JSON for the
+    workload when present (or --tuning PATH), the built-in values otherwise.
+    Set as the process default so every context (server workers included)
+    uses it.  Returns (tuning dict, source)."""
+    from trident_amd import autotune
+    from trident_amd.device import load_tuning, set_default_tuning, tuning_default
+    if args.tuning == "none":
+        return tuning_default(), "built-in"
+    path = autotune.default_path(args.workload) if args.tuning == "auto" else args.tuning
+    if not os.path.exists(path):
+        return tuning_default(), "built-in"
+    t = dict(tuning_default(), **load_tuning(path))
+    set_default_tuning(t)
+    return t, os.path.relpath(path, ROOT)
 
 
 def row_flops(A):
@@ -125,18 +139,13 @@ def stage_bench(args):
     products C_ij <- C_ij + A_ik * A_kj as one graph launch vs 16 plain
     sp_mma calls.  Partial-C tiles hold the same stage's products from a
     previous pass, so the accumulate merges equal structures."""
-    from trident_amd import gen
+    from trident_amd import workloads
     from trident_amd.device import Batch, Context, DeviceCsx, sp_mma, spgemm
     _, scale, k = args.workload.split(":")
     scale, k = int(scale), int(k)
     g = 4
     t0 = time.perf_counter()
-    A = gen.rmat(scale, 16, seed=2)
-    n = A.nrows // g
-    tiles = {}
-    for i in range(g):
-        tiles[(i, k)] = gen.submatrix(A, i * n, (i + 1) * n, k * n, (k + 1) * n)
-        tiles[(k, i)] = gen.submatrix(A, k * n, (k + 1) * n, i * n, (i + 1) * n)
+    tiles = workloads.stage_tiles(scale, k, g)
     log(f"[stage] tiles ready in {time.perf_counter() - t0:.1f}s")
     ctx = Context(0)
     dT = {key: ctx.upload(m) for key, m in tiles.items()}
@@ -227,7 +236,8 @@ def stage_bench(args):
         "steps": args.steps, "warmup": 1, "ms_per_step": res["batched"]["ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"R-MAT scale {scale} (ef 16, seed 2) cut 4x4; SUMMA stage k={k}: 16 tile products "
-                               "accumulated into partial-C tiles", "mults": mults, "forms": res},
+                               "accumulated into partial-C tiles", "mults": mults, "forms": res,
+                   "tuning": args.tuning_used, "tuning_source": args.tuning_source},
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -238,12 +248,11 @@ def galerkin_bench(args):
     smoothed-aggregation P (3^3 aggregates, damped Jacobi w = 2/3).  Plain:
     two tr_spgemm calls; batched: the two products as captured graphs (P^T
     from the on-device transpose, resident)."""
-    from trident_amd import gen
+    from trident_amd import workloads
     from trident_amd.device import Batch, Context, spgemm, spgemm_flops
     n = int(args.workload.split(":")[1])
     t0 = time.perf_counter()
-    A = gen.laplace27(n)
-    P = gen.sa_prolongator(n)
+    A, P = workloads.galerkin_operators(n)
     log(f"[galerkin] operators ready in {time.perf_counter() - t0:.1f}s: A {A.nrows} x {A.ncols} nnz {A.nnz}, "
         f"P {P.nrows} x {P.ncols} nnz {P.nnz}")
     ctx = Context(0)
@@ -297,7 +306,8 @@ def galerkin_bench(args):
            "steps": args.steps, "warmup": 1, "ms_per_step": res["batched"]["ms"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"C = P^T A P, A = 27-point Laplacian {n}^3 (26/-1), P = SA prolongator "
-                                  "(3^3 aggregates, unnormalised tentative P, w = 2/3)", "mults": mults, "forms": res}}
+                                  "(3^3 aggregates, unnormalised tentative P, w = 2/3)", "mults": mults, "forms": res,
+                      "tuning": args.tuning_used, "tuning_source": args.tuning_source}}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -400,7 +410,7 @@ def serve_bench(args):
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"{nreq} requests, Poisson arrivals: 60% A*A on R-MAT s12-18 (ef 8/16), 40% "
                                   "P^T A P on 27-pt Laplacians 16^3-96^3", "plain_GFLOP/s": round(plain_gflops, 2),
-                      "rates": results}}
+                      "rates": results, "tuning": args.tuning_used, "tuning_source": args.tuning_source}}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -444,19 +454,22 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
     ap.add_argument("--serve-workers", type=int, default=3)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.5, 1.0, 4.0])
+    ap.add_argument("--tuning", default="auto",
+                    help="auto (trident_amd/tuning/<workload>.json if present), none, or a JSON path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU's share of the path; --gpus {args.gpus} is not supported "
               "(run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.workload.startswith("copy:"):
+        return copy_bench(args)
+    args.tuning_used, args.tuning_source = apply_tuning(args)
     if args.workload.startswith("stage:"):
         return stage_bench(args)
     if args.workload.startswith("galerkin:"):
         return galerkin_bench(args)
     if args.workload.startswith("serve"):
         return serve_bench(args)
-    if args.workload.startswith("copy:"):
-        return copy_bench(args)
 
     from trident_amd.device import Context, spgemm_panels
 
@@ -517,7 +530,7 @@ def main():
     # dominant kernel and its algorithmic bytes per launch
     dom = max(kt.items(), key=lambda kv: kv[1][1]) if kt else ("none", (1, 0.0))
     dname, (dlaunch, dms) = dom
-    heavy = f > LIGHT_MAX_FLOPS
+    heavy = f > args.tuning_used["light_max_flops"]
     light = (f > 0) & ~heavy
     rl = np.diff(A.ptr.astype(np.int64))
     B_bytes = 12 * A.nnz + 4 * (A.nrows + 1)
@@ -582,6 +595,8 @@ def main():
             "nnz_C": nnzC,
             "panels": len(panels),
             "step_times_s": [round(x, 4) for x in times],
+            "tuning": args.tuning_used,
+            "tuning_source": args.tuning_source,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

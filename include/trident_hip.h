@@ -124,6 +124,34 @@ int tr_symbolic(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *nnz
 /* Number of multiplications sum_k nnz(A[:,k]) * nnz(B[k,:]) (device pass). */
 int tr_spgemm_flops(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *flops);
 
+/* ---- multiply tuning --------------------------------------------------------
+ * No counterpart in the reference: Kokkos-Kernels picks its spgemm algorithm
+ * internally (the handle of include/KokkosWrap.hpp:371-375 is created with
+ * its default).  The cutovers of this multiply are parameters, chosen per
+ * workload by trident_amd/autotune.py and loaded from its JSON output:
+ *  - light_max_flops: rows with more multiplications than this take the
+ *    heavy (column-windowed) path, [64, 1024];
+ *  - window_entries: heavy window packing quantum H (a window holds at most
+ *    H + 127 entries), [64, heavy_table / 2 - 127];
+ *  - item_windows: heavy windows per work item, [1, 64];
+ *  - heavy_table: slots of the heavy numeric table, 2048 (256-thread
+ *    workgroups) or 4096 (512-thread workgroups).
+ * Results do not depend on the tuning (bit-identical), only the time does.
+ * tr_tuning_default gives the built-in values; tr_set_default_tuning sets
+ * what contexts created afterwards start with (server workers included;
+ * NULL restores the built-in values); tr_ctx_set_tuning changes one context
+ * (NULL: the process default).  A batch keeps the tuning of its creation. */
+typedef struct tr_tuning {
+    int32_t light_max_flops;
+    int32_t window_entries;
+    int32_t item_windows;
+    int32_t heavy_table;
+} tr_tuning_t;
+int tr_tuning_default(tr_tuning_t *t);
+int tr_set_default_tuning(const tr_tuning_t *t);
+int tr_ctx_set_tuning(tr_ctx_t ctx, const tr_tuning_t *t);
+int tr_ctx_get_tuning(tr_ctx_t ctx, tr_tuning_t *t);
+
 /* ---- batched stage executor ------------------------------------------------ */
 
 /* One SUMMA / Trident stage of tile products on this GPU, run as a single

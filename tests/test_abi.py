@@ -50,3 +50,24 @@ def test_null_context_is_rejected_without_gpu():
     lib = _lib.load()
     assert lib.tr_spgemm(None, None, None, None) == _lib.TR_ERR_INVALID
     assert lib.tr_last_error(None) == b"null context"
+
+
+def test_tuning_defaults_and_validation_without_gpu():
+    from trident_amd import _lib
+    lib = _lib.load()
+    t = _lib.tr_tuning_t()
+    assert lib.tr_tuning_default(ctypes.byref(t)) == _lib.TR_OK
+    assert (t.light_max_flops, t.window_entries, t.item_windows, t.heavy_table) == (1024, 896, 16, 2048)
+    bad = [dict(light_max_flops=2048), dict(light_max_flops=32), dict(heavy_table=3000),
+           dict(window_entries=1000), dict(window_entries=32), dict(item_windows=0), dict(item_windows=65)]
+    for b in bad:
+        u = _lib.tr_tuning_t()
+        lib.tr_tuning_default(ctypes.byref(u))
+        for k, v in b.items():
+            setattr(u, k, v)
+        assert lib.tr_set_default_tuning(ctypes.byref(u)) == _lib.TR_ERR_INVALID, b
+    u = _lib.tr_tuning_t()
+    lib.tr_tuning_default(ctypes.byref(u))
+    u.heavy_table, u.window_entries = 4096, 1920
+    assert lib.tr_set_default_tuning(ctypes.byref(u)) == _lib.TR_OK
+    assert lib.tr_set_default_tuning(None) == _lib.TR_OK

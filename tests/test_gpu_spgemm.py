@@ -56,6 +56,29 @@ def test_rmat(ctx, scale, ef, seed):
     run(ctx, A, A, f"rmat s{scale}")
 
 
+TUNINGS = [
+    dict(heavy_table=4096, window_entries=1920),
+    dict(heavy_table=4096, window_entries=64, item_windows=1),
+    dict(window_entries=64, item_windows=64, light_max_flops=64),
+    dict(window_entries=384, item_windows=4, light_max_flops=256),
+]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
+def test_tuning_does_not_change_results(ctx, tuning):
+    # every cutover setting routes rows through different kernels; results
+    # stay bit-identical to the oracle
+    A = gen.rmat(13, 16, seed=6)
+    ctx.set_tuning(dict(ctx.get_tuning(), **tuning))
+    try:
+        run(ctx, A, A, f"tuning {tuning}")
+        B = gen.submatrix(gen.rmat(15, 16, seed=8), 0, 2000, 0, 32768)
+        C = gen.rmat(15, 16, seed=9)
+        run(ctx, B, C, f"tuning {tuning} wide")
+    finally:
+        ctx.set_tuning(None)
+
+
 def test_rmat_rectangular_pair(ctx):
     A = gen.submatrix(gen.rmat(13, 16, seed=4), 0, 3000, 0, 8192)
     B = gen.submatrix(gen.rmat(13, 16, seed=5), 0, 8192, 100, 5000)

@@ -46,6 +46,17 @@ class tr_csx_t(ctypes.Structure):
     ]
 
 
+class tr_tuning_t(ctypes.Structure):
+    """Mirror of `tr_tuning_t` (multiply cutovers chosen by the autotuner)."""
+
+    _fields_ = [
+        ("light_max_flops", ctypes.c_int32),
+        ("window_entries", ctypes.c_int32),
+        ("item_windows", ctypes.c_int32),
+        ("heavy_table", ctypes.c_int32),
+    ]
+
+
 # (name, restype, argtypes) for every entry point of include/trident_hip.h
 _P = ctypes.POINTER
 _csx = _P(tr_csx_t)
@@ -65,6 +76,10 @@ SIGNATURES = [
     ("tr_spadd", ctypes.c_int, [_vp, _csx, _csx]),
     ("tr_symbolic", ctypes.c_int, [_vp, _csx, _csx, _P(_i64), _P(_sz)]),
     ("tr_spgemm_flops", ctypes.c_int, [_vp, _csx, _csx, _P(_i64)]),
+    ("tr_tuning_default", ctypes.c_int, [_P(tr_tuning_t)]),
+    ("tr_set_default_tuning", ctypes.c_int, [_P(tr_tuning_t)]),
+    ("tr_ctx_set_tuning", ctypes.c_int, [_vp, _P(tr_tuning_t)]),
+    ("tr_ctx_get_tuning", ctypes.c_int, [_vp, _P(tr_tuning_t)]),
     ("tr_batch_create", ctypes.c_int, [_vp, _i32, _csx, _csx, _csx, _i32, _P(_vp)]),
     ("tr_batch_run", ctypes.c_int, [_vp]),
     ("tr_batch_result", ctypes.c_int, [_vp, _i32, _csx]),

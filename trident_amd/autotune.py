@@ -1,0 +1,242 @@
+"""Autotuner of the multiply cutovers (tr_tuning_t) on the GPU.
+
+The multiply has four cutovers, and none of them changes a result bit:
+
+- the flop count above which a row takes the heavy (column-windowed) path;
+- the window packing quantum H;
+- the number of windows per heavy work item;
+- the heavy table (2048 slots on 256 threads, or 4096 slots on 512).
+
+This tool times the workload's multiply under candidate settings and keeps
+the fastest. The search is coordinate descent from the built-in values:
+
+- the heavy table and its window quantum are searched together, then the
+  items, then the light/heavy cut;
+- a second sweep runs when the first one moved anything.
+
+It writes a JSON document whose "tuning" object `device.load_tuning`
+reads back. bench.py loads `trident_amd/tuning/<workload>.json` when it
+exists.
+
+  python -m trident_amd.autotune --workload rmat:20
+  python -m trident_amd.autotune --workload stage:22:3 --out my_tuning.json
+
+The timing covers the product as the benchmark runs it:
+
+- plain workloads: one tr_spgemm_panels call;
+- stage workloads: one replay of the captured batch;
+- galerkin: the two products.
+
+Each time is the median of `--reps` runs after one warm-up. Every candidate
+is also checked for identical output structure and a matching value
+fingerprint against the first run.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import hashlib
+import json
+import os
+import sys
+import time
+from typing import Callable, Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import workloads
+from .device import Batch, Context, spgemm, spgemm_panels, tuning_default
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNING_DIR = os.path.join(HERE, "tuning")
+
+# search space
+TABLE_WINDOWS = [(2048, 384), (2048, 640), (2048, 896), (4096, 1408), (4096, 1920)]
+ITEM_WINDOWS = [4, 8, 16, 32]
+LIGHT_MAX = [256, 512, 1024]
+
+
+def default_path(workload: str) -> str:
+    return os.path.join(TUNING_DIR, workload.replace(":", "_") + ".json")
+
+
+def _fingerprint(ctx: Context, mats) -> str:
+    """Digest of the row pointers and of up to 64 KiB of indices and values
+    of each result tile (a cheap identity check across candidates)."""
+    h = hashlib.sha256()
+    for d in mats:
+        n = d.d.nrows + 1
+        p = np.empty(n, np.int32)
+        ctx.check(ctx.lib.tr_memcpy_d2h(ctx.h, p.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(d.d.ptr_vec), 4 * n))
+        h.update(p.tobytes())
+        m = int(min(d.d.nnz, 8192))
+        if m:
+            idx = np.empty(m, np.int32)
+            val = np.empty(m, np.float64)
+            ctx.check(ctx.lib.tr_memcpy_d2h(ctx.h, idx.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(d.d.idx_vec),
+                                            4 * m))
+            ctx.check(ctx.lib.tr_memcpy_d2h(ctx.h, val.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(d.d.val),
+                                            8 * m))
+            h.update(idx.tobytes())
+            h.update(val.tobytes())
+    return h.hexdigest()
+
+
+def make_runner(ctx: Context, workload: str) -> Tuple[Callable[[], Tuple[float, str]], int]:
+    """Returns (run, mults): run() applies the context's current tuning,
+    executes the workload's multiply once untimed and `reps` times timed, and
+    returns (median seconds, output fingerprint)."""
+    from .device import spgemm_flops
+    if workload.startswith("stage:"):
+        _, s, k = workload.split(":")
+        tiles = workloads.stage_tiles(int(s), int(k))
+        dT = {key: ctx.upload(m) for key, m in tiles.items()}
+        keys = [(i, j) for i in range(4) for j in range(4)]
+        kk = int(k)
+        As = [dT[(i, kk)] for i, j in keys]
+        Bs = [dT[(kk, j)] for i, j in keys]
+        mults = sum(spgemm_flops(a, b) for a, b in zip(As, Bs))
+
+        def run(reps):
+            b = Batch(ctx, As, Bs, None)
+            b.run()
+            ctx.synchronize()
+            ts = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                b.run()
+                ctx.synchronize()
+                ts.append(time.perf_counter() - t)
+            fp = _fingerprint(ctx, [b.result(t) for t in range(len(keys))])
+            b.close()
+            return float(np.median(ts)), fp
+        return run, mults
+    if workload.startswith("galerkin:"):
+        A, P = workloads.galerkin_operators(int(workload.split(":")[1]))
+        dA, dP = ctx.upload(A), ctx.upload(P)
+        dPt = dP.transpose()
+        ap0 = spgemm(dA, dP)
+        mults = spgemm_flops(dA, dP) + spgemm_flops(dPt, ap0)
+        ap0.free()
+
+        def run(reps):
+            ts = []
+            out = None
+            for r in range(reps + 1):
+                t = time.perf_counter()
+                ap = spgemm(dA, dP)
+                c = spgemm(dPt, ap)
+                ctx.synchronize()
+                if r:
+                    ts.append(time.perf_counter() - t)
+                ap.free()
+                if out is not None:
+                    out.free()
+                out = c
+            fp = _fingerprint(ctx, [out])
+            out.free()
+            return float(np.median(ts)), fp
+        return run, mults
+    A = workloads.product_operand(workload)
+    dA = ctx.upload(A)
+    mults = spgemm_flops(dA, dA)
+
+    def run(reps):
+        ts = []
+        panels = []
+        for r in range(reps + 1):
+            for p in panels:
+                p.free()
+            t = time.perf_counter()
+            panels, _ = spgemm_panels(dA, dA, max_panels=1024)
+            ctx.synchronize()
+            if r:
+                ts.append(time.perf_counter() - t)
+        fp = _fingerprint(ctx, panels)
+        for p in panels:
+            p.free()
+        return float(np.median(ts)), fp
+    return run, mults
+
+
+def tune(workload: str, reps: int = 2, log=print) -> dict:
+    ctx = Context(0)
+    run, mults = make_runner(ctx, workload)
+    base = tuning_default()
+    seen: Dict[Tuple, float] = {}
+    records: List[dict] = []
+    ref_fp: Optional[str] = None
+
+    def measure(t: dict) -> float:
+        nonlocal ref_fp
+        key = tuple(t[f] for f in sorted(t))
+        if key in seen:
+            return seen[key]
+        ctx.set_tuning(t)
+        sec, fp = run(reps)
+        if ref_fp is None:
+            ref_fp = fp
+        ok = fp == ref_fp
+        seen[key] = sec if ok else float("inf")
+        records.append({"tuning": dict(t), "ms": round(1000 * sec, 3), "identical_output": ok})
+        log(f"[autotune] {workload} {t} -> {1000 * sec:.2f} ms{'' if ok else ' OUTPUT DIFFERS'}")
+        return seen[key]
+
+    best = dict(base)
+    best_t = measure(best)
+    default_t = best_t
+    for sweep in range(2):
+        moved = False
+        for table, h in TABLE_WINDOWS:
+            cand = dict(best, heavy_table=table, window_entries=h)
+            t = measure(cand)
+            if t < best_t:
+                best, best_t, moved = cand, t, True
+        for iw in ITEM_WINDOWS:
+            cand = dict(best, item_windows=iw)
+            t = measure(cand)
+            if t < best_t:
+                best, best_t, moved = cand, t, True
+        for lm in LIGHT_MAX:
+            cand = dict(best, light_max_flops=lm)
+            t = measure(cand)
+            if t < best_t:
+                best, best_t, moved = cand, t, True
+        if not moved:
+            break
+    props = {}
+    try:
+        import torch
+        props = {"device": torch.cuda.get_device_name(0)}
+    except Exception:
+        pass
+    return {
+        "workload": workload,
+        "tuning": best,
+        "best_ms": round(1000 * best_t, 3),
+        "default_ms": round(1000 * default_t, 3),
+        "speedup_vs_default": round(default_t / best_t, 4),
+        "mults": int(mults),
+        "reps": reps,
+        "candidates": records,
+        **props,
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--workload", default="rmat:20")
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--out", default=None, help="JSON output (default trident_amd/tuning/<workload>.json)")
+    args = ap.parse_args(argv)
+    doc = tune(args.workload, args.reps, log=lambda *a: print(*a, file=sys.stderr, flush=True))
+    out = args.out or default_path(args.workload)
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+    with open(out, "w") as f:
+        json.dump(doc, f, indent=1)
+    print(json.dumps({k: doc[k] for k in ("workload", "tuning", "best_ms", "default_ms", "speedup_vs_default")}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

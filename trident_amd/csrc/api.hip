@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 namespace tr {
@@ -226,6 +227,36 @@ static void copy_tile(tr_ctx *ctx, const tr_csx_t *src, tr_csx_t *dst) {
 }
 
 
+
+// ---------------------------------------------------------------- tuning
+tr_tuning_t tuning_builtin() {
+    tr_tuning_t t;
+    t.light_max_flops = 1024;
+    t.window_entries = 896;
+    t.item_windows = 16;
+    t.heavy_table = 2048;
+    return t;
+}
+
+namespace {
+std::mutex g_tune_mu;
+tr_tuning_t g_tune = tuning_builtin();
+}  // namespace
+
+tr_tuning_t tuning_process_default() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return g_tune;
+}
+
+void tuning_check(const tr_tuning_t &t) {
+    TR_REQUIRE(t.light_max_flops >= 64 && t.light_max_flops <= 1024,
+               "tuning: light_max_flops must be in [64, 1024]");
+    TR_REQUIRE(t.heavy_table == 2048 || t.heavy_table == 4096, "tuning: heavy_table must be 2048 or 4096");
+    TR_REQUIRE(t.window_entries >= 64 && t.window_entries <= t.heavy_table / 2 - 127,
+               "tuning: window_entries must be in [64, heavy_table / 2 - 127]");
+    TR_REQUIRE(t.item_windows >= 1 && t.item_windows <= 64, "tuning: item_windows must be in [1, 64]");
+}
+
 }  // namespace tr
 
 using namespace tr;
@@ -392,6 +423,37 @@ int tr_spgemm_flops(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t 
         TR_REQUIRE(flops != nullptr, "tr_spgemm_flops: null output");
         *flops = spgemm_flops(ctx, A, B);
     });
+}
+
+int tr_tuning_default(tr_tuning_t *t) {
+    if (!t) return TR_ERR_INVALID;
+    *t = tuning_builtin();
+    return TR_OK;
+}
+
+int tr_set_default_tuning(const tr_tuning_t *t) {
+    return guarded(nullptr, [&] {
+        const tr_tuning_t v = t ? *t : tuning_builtin();
+        tuning_check(v);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune = v;
+    });
+}
+
+int tr_ctx_set_tuning(tr_ctx_t ctx, const tr_tuning_t *t) {
+    TR_CTX_REQUIRED(ctx);
+    return guarded(ctx, [&] {
+        const tr_tuning_t v = t ? *t : tuning_process_default();
+        tuning_check(v);
+        ctx->tune = v;
+    });
+}
+
+int tr_ctx_get_tuning(tr_ctx_t ctx, tr_tuning_t *t) {
+    TR_CTX_REQUIRED(ctx);
+    if (!t) return TR_ERR_INVALID;
+    *t = ctx->tune;
+    return TR_OK;
 }
 
 int tr_bitmask_gen(tr_ctx_t ctx, const int32_t *ptr, int32_t n, uint64_t *mask) {

@@ -6,12 +6,12 @@
 //  1. row analysis   one wave per row: flops_i = sum_{k in A_i} nnz(B_k);
 //                    rows are binned by flops (empty / light classes by hash
 //                    table size / heavy).
-//  2. symbolic       light rows (flops <= kLightMaxF): one wave per row, LDS
+//  2. symbolic       light rows (flops <= the tuned cut): one wave per row, LDS
 //                    open-addressing hash of column keys;
 //                    heavy rows: one 1024-thread block per row, a bitmap of
 //                    2^20 columns in LDS (128 KiB of the 160 KiB), popcounted
 //                    per 128-column fine block; the block packs fine blocks
-//                    into column windows of at most kWinH+127 entries and
+//                    into column windows of at most H+127 entries and
 //                    appends them to a device work list.
 //  3. scan           row counts -> 64-bit row offsets; C is cut into int32
 //                    tiles (panels) when it exceeds INT32_MAX entries.
@@ -146,7 +146,7 @@ __global__ void k_zero_i64(int64_t *__restrict__ p, int64_t n) {
 __global__ __launch_bounds__(kRowsPerBlock) void k_row_flops(const TaskDev *__restrict__ tasks, int nt,
                                                              int32_t rows_total, int64_t *__restrict__ flops,
                                                              int32_t *__restrict__ lists,
-                                                             Counters *__restrict__ cnt) {
+                                                             Counters *__restrict__ cnt, int32_t light_max) {
     __shared__ int32_t s_cnt[kNumClasses], s_base[kNumClasses];
     __shared__ unsigned long long s_f, s_hf;
     const int tid = threadIdx.x, l = lane_id();
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_row_flops(const TaskDev *__re
     }
     int c = -1, lpos = 0;
     if (g < rows_total && f > 0) {
-        if (f <= kLightMaxF) {
+        if (f <= light_max) {
             const int64_t bound = f < ncols ? f : ncols;
             int64_t T = 64;
             c = 0;
@@ -279,8 +279,8 @@ __global__ __launch_bounds__(256) void k_sym_light(const TaskDev *__restrict__ t
 // appended to the device work lists.
 __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
     const TaskDev *__restrict__ tasks, int nt, const int32_t *__restrict__ rows, Counters *__restrict__ cnt,
-    int64_t *__restrict__ rownnz, int emit, int32_t win_h, Window *__restrict__ wins, int32_t win_cap,
-    Item *__restrict__ items, int32_t item_cap) {
+    int64_t *__restrict__ rownnz, int emit, int32_t win_h, int32_t item_windows, Window *__restrict__ wins,
+    int32_t win_cap, Item *__restrict__ items, int32_t item_cap) {
     constexpr int kWords = 1 << (kPassBits - 5);          // 32768 words
     constexpr int kFine = 1 << (kPassBits - kFineShift);  // 8192 fine blocks
     constexpr int kPerThread = kFine / kHeavyThreads;     // 8
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
             for (int u = 0; u < kPerThread; ++u)
                 if (flags & (1 << u)) first_lds[rk++] = tid * kPerThread + u;
             if (tid == 0) {
-                const int32_t nitems = (nwin + kItemWindows - 1) / kItemWindows;
+                const int32_t nitems = (nwin + item_windows - 1) / item_windows;
                 const int32_t wb = atomicAdd(&cnt->windows, nwin);
                 const int32_t ib = atomicAdd(&cnt->items, nitems);
                 if (wb + nwin > win_cap || ib + nitems > item_cap) {
@@ -410,8 +410,8 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
                     for (int32_t it = 0; it < nitems; ++it) {
                         Item I;
                         I.row = g;
-                        I.wbeg = wb + it * kItemWindows;
-                        I.wend = min(wb + nwin, I.wbeg + kItemWindows);
+                        I.wbeg = wb + it * item_windows;
+                        I.wend = min(wb + nwin, I.wbeg + item_windows);
                         I.pad = 0;
                         items[ib + it] = I;
                     }
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(256) void k_num_light(const TaskDev *__restrict__ t
 }
 
 // ---------------------------------------------------------------- numeric, heavy rows (workgroup)
-// One NT-thread workgroup per item (<= kItemWindows windows of one row),
+// One NT-thread workgroup per item (a run of windows of one row),
 // sharing one T-slot LDS table per window.  For each window the row's A
 // entries are taken NT at a time (one per thread): each thread finds the
 // part of B_k inside the window (register cursor for rows with <= NT
@@ -608,7 +608,13 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
     if (tid == 0) L.nocc = 0;
     __syncthreads();
     const int32_t nitems = cnt->items;
-    for (int32_t itx = blockIdx.x; itx < nitems; itx += gridDim.x) {
+    // Workgroups are dispatched round-robin over the 8 XCDs; each XCD takes a
+    // contiguous run of items, so the items of one row (which read the same
+    // B rows) share that XCD's L2.  gridDim.x is a multiple of 8.
+    const int32_t per_xcd = (int32_t)(gridDim.x >> 3);
+    const int32_t slot = TR_XCD_MAP ? (int32_t)(blockIdx.x & 7) * per_xcd + (int32_t)(blockIdx.x >> 3)
+                                    : (int32_t)blockIdx.x;
+    for (int32_t itx = slot; itx < nitems; itx += gridDim.x) {
         const Item I = items[itx];
         const int32_t g = I.row;
         const TaskDev &TK = tasks[task_of(tasks, nt, g)];
@@ -831,6 +837,8 @@ uint32_t cap_grid(int64_t need, int64_t cap) {
     return (uint32_t)std::min<int64_t>(need, cap);
 }
 
+uint32_t round_up8(uint32_t g) { return (g + 7u) & ~7u; }
+
 template <int T>
 void launch_sym_light(tr_ctx *ctx, MulPlan &pl, int c) {
     const int32_t n = pl.h.cls[c];
@@ -850,6 +858,7 @@ void launch_num_light(tr_ctx *ctx, MulPlan &pl, int c, Panel *panels, int np) {
 void mul_setup(tr_ctx *ctx, Workspace &ws, MulPlan &pl, const std::vector<TaskDev> &tasks) {
     pl.nt = (int)tasks.size();
     pl.h_tasks = tasks;
+    pl.tune = ctx->tune;
     int64_t rows = 0;
     for (auto &t : pl.h_tasks) {
         t.row_base = (int32_t)rows;
@@ -874,7 +883,8 @@ void mul_analyze(tr_ctx *ctx, MulPlan &pl) {
     TR_LAUNCH(ctx, "zero_rownnz", k_zero_i64, cap_grid(div_up((uint64_t)pl.rows_total + 1, 256), 4096), 256, 0,
               pl.rownnz, (int64_t)pl.rows_total + 1);
     TR_LAUNCH(ctx, "row_flops", k_row_flops, div_up((uint64_t)pl.rows_total, kRowsPerBlock), kRowsPerBlock, 0,
-              pl.d_tasks, pl.nt, pl.rows_total, pl.flops, pl.lists, pl.cnt);
+              pl.d_tasks, pl.nt, pl.rows_total, pl.flops, pl.lists, pl.cnt,
+              pl.tune.light_max_flops);
 }
 
 void mul_read_counts(tr_ctx *ctx, MulPlan &pl) { read_array(ctx, &pl.h, pl.cnt, sizeof(Counters)); }
@@ -894,8 +904,9 @@ void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool captu
             // windows per pass <= ceil(pass_nnz / H) + width splits
             const int64_t passes = ((int64_t)max_n + (1 << kPassBits) - 1) >> kPassBits;
             const int64_t wsplits = (int64_t)max_n / kWinMaxWidth + 1;
-            const int64_t wcap = (int64_t)(pl.h.heavy_flops / kWinH) + (int64_t)nh * (2 * passes + wsplits) + 16;
-            const int64_t icap = wcap / kItemWindows + (int64_t)nh * passes + 16;
+            const int64_t wcap =
+                (int64_t)(pl.h.heavy_flops / pl.tune.window_entries) + (int64_t)nh * (2 * passes + wsplits) + 16;
+            const int64_t icap = wcap / pl.tune.item_windows + (int64_t)nh * passes + 16;
             TR_REQUIRE(wcap < INT32_MAX && icap < INT32_MAX, "spgemm: heavy work list exceeds int32 capacity");
             pl.win_cap = (int32_t)wcap;
             pl.item_cap = (int32_t)icap;
@@ -904,7 +915,8 @@ void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool captu
         }
         TR_LAUNCH(ctx, "sym_heavy", k_sym_heavy, cap_grid(nh, (int64_t)ctx->cu_count * 2), kHeavyThreads, 0,
                   pl.d_tasks, pl.nt, pl.lists + (int64_t)kHeavyClass * pl.rows_total, pl.cnt, pl.rownnz,
-                  emit ? 1 : 0, (int32_t)kWinH, pl.wins, pl.win_cap, pl.items, pl.item_cap);
+                  emit ? 1 : 0, pl.tune.window_entries, pl.tune.item_windows, pl.wins, pl.win_cap, pl.items,
+                  pl.item_cap);
     }
     launch_exclusive_scan_i64(ctx, pl.rownnz, pl.rowptr, pl.rows_total, pl.scan_tmp);
 }
@@ -935,9 +947,13 @@ void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *dp, int np) {
     if (pl.h.cls[4]) launch_num_light<1024>(ctx, pl, 4, dp, np);
     if (pl.h.cls[5]) launch_num_light<2048>(ctx, pl, 5, dp, np);
     if (pl.h.cls[kHeavyClass] && pl.h.items > 0) {
-        TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<kN2T, kHeavyWgThreads>), cap_grid(pl.h.items, 1 << 24),
-                  kHeavyWgThreads, 0, pl.d_tasks, pl.nt, pl.items, pl.cnt, pl.wins, pl.cbase, pl.cursors, pl.rowptr,
-                  dp, np);
+        const uint32_t grid = round_up8(cap_grid(pl.h.items, 1 << 24));
+        if (pl.tune.heavy_table == 4096)
+            TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<4096, 512>), grid, 512, 0, pl.d_tasks, pl.nt, pl.items,
+                      pl.cnt, pl.wins, pl.cbase, pl.cursors, pl.rowptr, dp, np);
+        else
+            TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<2048, 256>), grid, 256, 0, pl.d_tasks, pl.nt, pl.items,
+                      pl.cnt, pl.wins, pl.cbase, pl.cursors, pl.rowptr, dp, np);
     }
 }
 
@@ -1004,7 +1020,7 @@ void spgemm_symbolic_only(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int
     if (bytes) {
         // output tile + work lists + cursors bound + row arrays
         const size_t out = (size_t)total * 12 + ((size_t)pl.rows_total + 1) * 4;
-        const size_t heavy = (size_t)(pl.h.heavy_flops / kWinH + 16) * (sizeof(Window) + sizeof(Item));
+        const size_t heavy = (size_t)(pl.h.heavy_flops / pl.tune.window_entries + 16) * (sizeof(Window) + sizeof(Item));
         const size_t rows = ((size_t)pl.rows_total + 1) * (8 * 3 + 4 * kNumClasses);
         *bytes = out + heavy + rows + (size_t)A->nnz * 4;
     }

@@ -5,31 +5,23 @@
 #include <cstdint>
 #include <vector>
 
-#ifndef TR_HEAVY_T
-#define TR_HEAVY_T 2048
-#endif
-#ifndef TR_WIN_H
-#define TR_WIN_H 896
+#ifndef TR_XCD_MAP
+#define TR_XCD_MAP 1
 #endif
 
 namespace tr {
 
 // tuning constants (see DESIGN notes in the repository documentation)
-constexpr int kLightMaxF = 1024;   // symbolic light/heavy cut on row flops
+constexpr int kLightMaxFCap = 1024; // largest light/heavy cut (light tables reach 2048 slots)
 constexpr int kNumLight = 6;       // light classes: T = 64 << c, c < kNumLight
 constexpr int kHeavyClass = kNumLight;
 constexpr int kNumClasses = kNumLight + 1;
 constexpr int kPassBits = 20;      // heavy bitmap covers 2^20 columns per pass
 constexpr int kFineShift = 7;      // fine block = 128 columns
-constexpr int kWinH = TR_WIN_H;     // window packing quantum (entries)
-constexpr int kN2T = TR_HEAVY_T;   // heavy window hash slots (>= 2 * (kWinH + 127))
-static_assert(2 * (kWinH + 127) <= kN2T, "heavy table must hold a window at load <= 1/2");
 constexpr int kWinMaxWidth = 32768;                       // window width cap (columns)
 constexpr int kWidthShift = 15 - kFineShift;               // fine blocks per width quantum: 2^8
 constexpr int kBmWords = kWinMaxWidth / 64;                // presence words per window
-constexpr int kItemWindows = 16;   // windows per heavy work item
 constexpr int kHeavyThreads = 1024;  // symbolic heavy block
-constexpr int kHeavyWgThreads = 256;  // numeric heavy workgroup
 
 struct Window {
     int32_t c0, c1;   // column range [c0, c1)
@@ -112,6 +104,7 @@ struct MulPlan {
     int64_t cursor_cap = 0;
     int64_t *scan_tmp = nullptr;  // owned scan temp (graph mode)
     size_t scan_tmp_bytes = 0;
+    tr_tuning_t tune{};           // the context's tuning when the plan was set up
 };
 
 // One add X_t + Y_t of a batch.  X rows come from xp (int32, per task) or,

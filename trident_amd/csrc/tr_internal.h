@@ -71,8 +71,16 @@ struct KernelStat {
 
 }  // namespace tr
 
+namespace tr {
+// built-in multiply tuning (see tr_tuning_t) and its validation
+tr_tuning_t tuning_builtin();
+tr_tuning_t tuning_process_default();
+void tuning_check(const tr_tuning_t &t);  // throws TR_ERR_INVALID
+}  // namespace tr
+
 struct tr_ctx {
     int device = 0;
+    tr_tuning_t tune = tr::tuning_process_default();
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string last_error;

@@ -23,6 +23,45 @@ def _np_ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
+TUNING_FIELDS = ("light_max_flops", "window_entries", "item_windows", "heavy_table")
+
+
+def _tuning_struct(tuning: dict) -> "_lib.tr_tuning_t":
+    lib = _lib.load()
+    t = _lib.tr_tuning_t()
+    lib.tr_tuning_default(ctypes.byref(t))
+    for k, v in tuning.items():
+        if k not in TUNING_FIELDS:
+            raise ValueError(f"unknown tuning field {k!r}")
+        setattr(t, k, int(v))
+    return t
+
+
+def tuning_default() -> dict:
+    """The library's built-in multiply tuning."""
+    t = _lib.tr_tuning_t()
+    _lib.load().tr_tuning_default(ctypes.byref(t))
+    return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
+
+
+def set_default_tuning(tuning: Optional[dict]):
+    """Tuning that contexts created afterwards start with (None: built-in)."""
+    lib = _lib.load()
+    t = _tuning_struct(tuning) if tuning is not None else None
+    rc = lib.tr_set_default_tuning(ctypes.byref(t) if t is not None else None)
+    if rc != _lib.TR_OK:
+        raise TridentError(rc, f"invalid tuning {tuning}")
+
+
+def load_tuning(path: str) -> dict:
+    """The tuning chosen by trident_amd.autotune, from its JSON output."""
+    import json
+    with open(path) as f:
+        doc = json.load(f)
+    t = doc.get("tuning", doc)
+    return {k: int(t[k]) for k in TUNING_FIELDS if k in t}
+
+
 class Context:
     """A device context (one HIP stream, one caching pool)."""
 
@@ -59,6 +98,16 @@ class Context:
 
     def synchronize(self):
         self.check(self.lib.tr_ctx_synchronize(self.h), "synchronize")
+
+    def set_tuning(self, tuning: Optional[dict]):
+        """Multiply cutovers of this context (None: the process default)."""
+        t = _tuning_struct(tuning) if tuning is not None else None
+        self.check(self.lib.tr_ctx_set_tuning(self.h, ctypes.byref(t) if t is not None else None), "tuning")
+
+    def get_tuning(self) -> dict:
+        t = _lib.tr_tuning_t()
+        self.check(self.lib.tr_ctx_get_tuning(self.h, ctypes.byref(t)), "tuning")
+        return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
 
     @property
     def stream(self) -> int:
