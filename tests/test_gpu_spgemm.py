@@ -137,6 +137,29 @@ def test_heavy_multipass_wide(ctx):
     run(ctx, A, B, "multipass")
 
 
+@pytest.mark.parametrize("ncols", [1 << 19, (1 << 20) + 777])
+def test_heavy_sparse_rows(ctx, ncols):
+    # heavy rows whose few thousand outputs spread over all columns: their
+    # column passes are sparse, so windows are cut by count only (wider than
+    # the bitmap) and ranked by key comparison; dense rows in the same matrix
+    # keep bitmap windows; overlapping B rows give collisions inside windows
+    rng = np.random.default_rng(11)
+    k = 400
+    bl = []
+    for r in range(k):
+        cnt = 3000 if r % 50 == 0 else int(rng.integers(30, 200))
+        bl.append(np.sort(rng.choice(ncols, size=cnt, replace=False)))
+    bptr = np.concatenate([[0], np.cumsum([len(x) for x in bl])])
+    B = gen.from_arrays(k, ncols, bptr, np.concatenate(bl), rng.uniform(-1, 1, bptr[-1]))
+    al = []
+    for i in range(96):
+        size = int(rng.integers(8, 60)) if i % 3 else int(rng.integers(100, 300))
+        al.append(np.sort(rng.choice(k, size=size, replace=False)))
+    aptr = np.concatenate([[0], np.cumsum([len(x) for x in al])])
+    A = gen.from_arrays(96, k, aptr, np.concatenate(al), rng.uniform(-1, 1, aptr[-1]))
+    run(ctx, A, B, f"sparse heavy {ncols}")
+
+
 def test_structural_zero_and_unsorted_A(ctx):
     # products cancelling to exactly 0 keep their entry; A's stored order
     # (unsorted within a row) defines the accumulation order

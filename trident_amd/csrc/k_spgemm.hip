@@ -19,11 +19,12 @@
 //                    A's entries walked in stored order with the lanes over
 //                    B_k (no two lanes share a column, so values need no
 //                    atomics and every output sees its products in order);
-//                    heavy rows: one wave per group of windows, products of
-//                    the window flattened over the lanes in A-order, column
-//                    conflicts inside a step applied in lane (= A) order.
-//                    Each row/window is compacted and bitonic-sorted in LDS
-//                    and streamed out.
+//                    light rows are compacted and bitonic-sorted in LDS;
+//                    heavy rows: one workgroup per item (a run of windows of
+//                    one row) sharing one LDS table per window, products of
+//                    the window flattened over the workgroup in A-order,
+//                    column conflicts inside a step applied in thread (= A)
+//                    order, sorted ranks from a presence bitmap.
 //
 // Every output entry therefore receives c = fma(a_ik, b_kj, c) in the order
 // of A's stored entries, starting from +0.0: the results are bit-identical to
@@ -90,6 +91,34 @@ __device__ __forceinline__ void stream_cols(const int32_t *__restrict__ bi, int3
         f(j3);
     }
     for (; q < e; q += kWave) f(bi[q]);
+}
+
+// Same visit with 16-byte loads for the aligned body of the segment (four
+// columns per lane per load, four loads in flight), scalar head and tail.
+// Alignment is taken from the address: caller tiles may place indices at any
+// 4-byte offset.
+template <typename F>
+__device__ __forceinline__ void stream_cols_v4(const int32_t *__restrict__ bi, int32_t s, int32_t e, F &&f) {
+    const int l = lane_id();
+    const int32_t mis = (int32_t)(((uintptr_t)(bi + s) >> 2) & 3);  // dwords past a 16-byte boundary
+    const int32_t sa = min(e, s + ((4 - mis) & 3));                  // first 16-byte aligned index
+    if (s + l < sa) f(bi[s + l]);
+    const int32_t nv = (e - sa) >> 2;                                // whole int4 groups
+    const int4 *__restrict__ b4 = reinterpret_cast<const int4 *>(bi + sa);
+    int32_t v = l;
+    for (; v + 3 * kWave < nv; v += 4 * kWave) {
+        const int4 x0 = b4[v], x1 = b4[v + kWave], x2 = b4[v + 2 * kWave], x3 = b4[v + 3 * kWave];
+        f(x0.x), f(x0.y), f(x0.z), f(x0.w);
+        f(x1.x), f(x1.y), f(x1.z), f(x1.w);
+        f(x2.x), f(x2.y), f(x2.z), f(x2.w);
+        f(x3.x), f(x3.y), f(x3.z), f(x3.w);
+    }
+    for (; v < nv; v += kWave) {
+        const int4 x = b4[v];
+        f(x.x), f(x.y), f(x.z), f(x.w);
+    }
+    const int32_t ta = sa + (nv << 2);
+    if (ta + l < e) f(bi[ta + l]);
 }
 
 // Same with the values (all entries of one B row have distinct columns, so
@@ -322,7 +351,7 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
                 const int nk = rem <= 0 ? 0 : min(kWave, (rem + kWavesB - 1) / kWavesB);
                 for (int t = 0; t < nk; ++t) {
                     const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
-                    stream_cols(bi, s, e, [&](int32_t j) {
+                    stream_cols_v4(bi, s, e, [&](int32_t j) {
                         const int64_t jj = (int64_t)j - base;
                         if (jj >= 0 && jj < width) atomicOr(&bm[jj >> 5], 1u << (jj & 31));
                     });
@@ -371,15 +400,20 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
 #pragma unroll
             for (int u = 0; u < kPerThread; ++u) fex_lds[tid * kPerThread + u] = fex[u];
             __syncthreads();
-            // window starts: a count quantum or a width quantum begins
+            // window starts: a count quantum or a width quantum begins.  A
+            // sparse pass (few entries per width quantum) is cut by a smaller
+            // count quantum only; its windows may be as wide as the pass and
+            // are ranked without a bitmap by the numeric kernel.
+            const bool sparse_pass = (int64_t)pass_total * (1 << kWidthShift) <= (int64_t)kSparseAvg * nfine;
+            const int32_t q = sparse_pass ? kSparseH : win_h;
             int flags = 0, nflag = 0;
 #pragma unroll
             for (int u = 0; u < kPerThread; ++u) {
                 const int f = tid * kPerThread + u;
                 bool first = false;
                 if (f < nfine)
-                    first = (f == 0) || (fex[u] / win_h != fex_lds[f - 1] / win_h) ||
-                            ((f >> kWidthShift) != ((f - 1) >> kWidthShift));
+                    first = (f == 0) || (fex[u] / q != fex_lds[f - 1] / q) ||
+                            (!sparse_pass && ((f >> kWidthShift) != ((f - 1) >> kWidthShift)));
                 if (first) {
                     flags |= 1 << u;
                     ++nflag;
@@ -548,7 +582,8 @@ __global__ __launch_bounds__(256) void k_num_light(const TaskDev *__restrict__ t
 // A-order) through atomicMin rounds.  A presence bitmap over the window gives
 // every entry its sorted rank (prefix popcounts) for the write-out.
 constexpr uint32_t kSlotFree = 0xffffffffu;
-constexpr uint32_t kNoTag = 0xffffu;
+constexpr int kTagBits = 10;                        // thread ids < 1023
+constexpr uint32_t kNoTag = (1u << kTagBits) - 1u;  // slot key = column offset << kTagBits | tag
 
 template <int T, int NT>
 struct HeavyWgLds {
@@ -647,6 +682,10 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             if (W.cnt == 0) continue;  // block-uniform
             const bool full = (W.c0 == 0 && W.c1 >= ncols);
             const bool last = (wi == I.wend - 1);
+            // windows up to kWinMaxWidth columns are ranked by the presence
+            // bitmap; wider (sparse) windows hold <= kSparseH + 127 entries
+            // and are ranked by counting smaller keys
+            const bool dense = (W.c1 - W.c0) <= kWinMaxWidth;
             int nfresh = 0;
             for (int32_t pb = a0; pb < a1; pb += NT) {
                 // this thread's k-slot: A entry p, B segment [s, e) inside the window
@@ -710,19 +749,19 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                     uint32_t h = 0, key = 0;
                     if (act) {
                         const uint32_t off = (uint32_t)(j - W.c0);
-                        key = off << 16;
+                        key = off << kTagBits;
                         h = hash_slot((int32_t)off, kLog);
                         bool placed = false;
                         for (int probe = 0; probe < T; ++probe) {
                             const uint32_t old = atomicCAS(&L.keytag[h], kSlotFree, key | kNoTag);
                             if (old == kSlotFree) {
                                 L.vals[h] = 0.0;
-                                atomicOr(&L.bm[off >> 6], 1ull << (off & 63));
+                                if (dense) atomicOr(&L.bm[off >> 6], 1ull << (off & 63));
                                 ++nfresh;
                                 placed = true;
                                 break;
                             }
-                            if ((old >> 16) == off) {
+                            if ((old >> kTagBits) == off) {
                                 placed = true;
                                 break;
                             }
@@ -757,43 +796,76 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             // entries created in this window (each thread counted its fresh inserts)
             const int32_t wf = wave_sum(nfresh);
             if (l == 0 && wf) atomicAdd(&L.nocc, wf);
-            // prefix popcounts of the presence words
-            const int nwords = (W.c1 - W.c0 + 63) >> 6;
-            constexpr int kPer = kBmWords / NT > 0 ? kBmWords / NT : 1;
-            int32_t wc[kPer];
-            int32_t tsum = 0;
-#pragma unroll
-            for (int u = 0; u < kPer; ++u) {
-                const int wd = tid * kPer + u;
-                wc[u] = (wd < nwords && wd < kBmWords) ? __popcll(L.bm[wd]) : 0;
-                tsum += wc[u];
-            }
-            int32_t tot;
-            int32_t run = wg_excl_scan<NT>(tsum, L.wsum, &tot);
-#pragma unroll
-            for (int u = 0; u < kPer; ++u) {
-                const int wd = tid * kPer + u;
-                if (wd < kBmWords) L.wpre[wd] = (uint16_t)run;
-                run += wc[u];
-            }
-            __syncthreads();
-            if (tid == 0 && L.nocc != W.cnt) atomicOr(&cnt->overflow, 4);
             int32_t *oi = oi_row + W.off;
             double *ov = ov_row + W.off;
-            for (int h = tid; h < T; h += NT) {
-                const uint32_t kt = L.keytag[h];
-                if (kt == kSlotFree) continue;
-                const uint32_t off = kt >> 16;
-                const int wd = (int)(off >> 6);
-                const int32_t rr = (int32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
-                if (rr < W.cnt) {  // never write past the window
-                    oi[rr] = W.c0 + (int32_t)off;
-                    ov[rr] = L.vals[h];
+            const int nwords = dense ? (W.c1 - W.c0 + 63) >> 6 : kBmWords;
+            if (dense) {
+                // prefix popcounts of the presence words
+                constexpr int kPer = kBmWords / NT > 0 ? kBmWords / NT : 1;
+                int32_t wc[kPer];
+                int32_t tsum = 0;
+#pragma unroll
+                for (int u = 0; u < kPer; ++u) {
+                    const int wd = tid * kPer + u;
+                    wc[u] = (wd < nwords && wd < kBmWords) ? __popcll(L.bm[wd]) : 0;
+                    tsum += wc[u];
                 }
+                int32_t tot;
+                int32_t run = wg_excl_scan<NT>(tsum, L.wsum, &tot);
+#pragma unroll
+                for (int u = 0; u < kPer; ++u) {
+                    const int wd = tid * kPer + u;
+                    if (wd < kBmWords) L.wpre[wd] = (uint16_t)run;
+                    run += wc[u];
+                }
+                __syncthreads();
+                if (tid == 0 && L.nocc != W.cnt) atomicOr(&cnt->overflow, 4);
+                for (int h = tid; h < T; h += NT) {
+                    const uint32_t kt = L.keytag[h];
+                    if (kt == kSlotFree) continue;
+                    const uint32_t off = kt >> kTagBits;
+                    const int wd = (int)(off >> 6);
+                    const int32_t rr = (int32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
+                    if (rr < W.cnt) {  // never write past the window
+                        oi[rr] = W.c0 + (int32_t)off;
+                        ov[rr] = L.vals[h];
+                    }
+                }
+            } else {
+                // compact the occupied slots (keys into the bitmap's space,
+                // slot numbers into the prefix array), then rank each entry by
+                // the number of smaller keys
+                uint32_t *lkey = reinterpret_cast<uint32_t *>(L.bm);
+                uint16_t *lslot = L.wpre;
+                int32_t mine = 0;
+                for (int h = tid; h < T; h += NT) mine += (L.keytag[h] != kSlotFree) ? 1 : 0;
+                int32_t tot;
+                int32_t pos = wg_excl_scan<NT>(mine, L.wsum, &tot);
+                const bool fits = tot <= kSparseH + 127;  // bound of a sparse window
+                if (fits)
+                    for (int h = tid; h < T; h += NT) {
+                        const uint32_t kt = L.keytag[h];
+                        if (kt == kSlotFree) continue;
+                        lkey[pos] = kt >> kTagBits;
+                        lslot[pos] = (uint16_t)h;
+                        ++pos;
+                    }
+                __syncthreads();
+                if (tid == 0 && (L.nocc != W.cnt || !fits)) atomicOr(&cnt->overflow, 4);
+                if (fits)
+                    for (int e = tid; e < tot; e += NT) {
+                        const uint32_t k = lkey[e];
+                        int32_t rr = 0;
+                        for (int u = 0; u < tot; ++u) rr += lkey[u] < k ? 1 : 0;
+                        if (rr < W.cnt) {
+                            oi[rr] = W.c0 + (int32_t)k;
+                            ov[rr] = L.vals[lslot[e]];
+                        }
+                    }
             }
             __syncthreads();
             for (int t = tid; t < T; t += NT) L.keytag[t] = kSlotFree;
-            for (int t = tid; t < nwords; t += NT) L.bm[t] = 0ull;
+            for (int t = tid; t < nwords; t += NT) L.bm[t] = 0ull;  // also the key list of a sparse window
             if (tid == 0) L.nocc = 0;
             __syncthreads();
         }
@@ -904,8 +976,8 @@ void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool captu
             // windows per pass <= ceil(pass_nnz / H) + width splits
             const int64_t passes = ((int64_t)max_n + (1 << kPassBits) - 1) >> kPassBits;
             const int64_t wsplits = (int64_t)max_n / kWinMaxWidth + 1;
-            const int64_t wcap =
-                (int64_t)(pl.h.heavy_flops / pl.tune.window_entries) + (int64_t)nh * (2 * passes + wsplits) + 16;
+            const int64_t q = std::min<int64_t>(pl.tune.window_entries, kSparseH);
+            const int64_t wcap = (int64_t)(pl.h.heavy_flops / q) + (int64_t)nh * (2 * passes + wsplits) + 16;
             const int64_t icap = wcap / pl.tune.item_windows + (int64_t)nh * passes + 16;
             TR_REQUIRE(wcap < INT32_MAX && icap < INT32_MAX, "spgemm: heavy work list exceeds int32 capacity");
             pl.win_cap = (int32_t)wcap;
@@ -1020,7 +1092,8 @@ void spgemm_symbolic_only(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int
     if (bytes) {
         // output tile + work lists + cursors bound + row arrays
         const size_t out = (size_t)total * 12 + ((size_t)pl.rows_total + 1) * 4;
-        const size_t heavy = (size_t)(pl.h.heavy_flops / pl.tune.window_entries + 16) * (sizeof(Window) + sizeof(Item));
+        const size_t heavy = (size_t)(pl.h.heavy_flops / std::min<int64_t>(pl.tune.window_entries, kSparseH) + 16) *
+                             (sizeof(Window) + sizeof(Item));
         const size_t rows = ((size_t)pl.rows_total + 1) * (8 * 3 + 4 * kNumClasses);
         *bytes = out + heavy + rows + (size_t)A->nnz * 4;
     }

@@ -21,6 +21,8 @@ constexpr int kFineShift = 7;      // fine block = 128 columns
 constexpr int kWinMaxWidth = 32768;                       // window width cap (columns)
 constexpr int kWidthShift = 15 - kFineShift;               // fine blocks per width quantum: 2^8
 constexpr int kBmWords = kWinMaxWidth / 64;                // presence words per window
+constexpr int kSparseAvg = 192;    // a pass with <= this many entries per width quantum is sparse
+constexpr int kSparseH = 256;      // sparse windows: count quantum (<= kSparseH + 127 entries, any width)
 constexpr int kHeavyThreads = 1024;  // symbolic heavy block
 
 struct Window {
