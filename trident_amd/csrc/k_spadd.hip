@@ -6,11 +6,10 @@
 // 799-905: csrgeam2 bufferSize / Nnz / compute with a host sync after each).
 //
 // Merge-path formulation: every row's union is cut into chunks of kChunk
-// merged positions; a wave owns a chunk, splits its two ends by binary
-// search in global memory (ties take X's entry first, so an equal column
-// appears as an adjacent (X, Y) pair that collapses into one output entry
-// x + y), stages the chunk's X and Y ranges in LDS and merges them there,
-// kChunk / 64 positions per lane.  Pass 1 counts the kept positions per
+// merged positions; a wave owns a chunk, splits its start by binary search
+// in global memory (ties take X's entry first, so an equal column appears as
+// an adjacent (X, Y) pair that collapses into one output entry x + y) and
+// walks it in LDS-staged sub-steps.  Pass 1 counts the kept positions per
 // chunk, one scan over all chunks gives both the row pointers and every
 // chunk's output offset, pass 2 writes.  Long rows are spread over many
 // waves.
@@ -29,7 +28,7 @@ namespace {
 
 using namespace dev;
 
-constexpr int kChunk = 256;
+constexpr int kChunk = 4096;  // merged positions per wave work unit
 
 __device__ __forceinline__ int add_task_of(const AddTaskDev *__restrict__ tasks, int nt, int32_t g) {
     int lo = 0, hi = nt - 1;
@@ -89,17 +88,21 @@ __device__ __forceinline__ int32_t merge_split(const int32_t *__restrict__ xi, i
     return lo;
 }
 
-constexpr int kPerLane = kChunk / kWave;  // merged positions per lane
-constexpr int32_t kGuardLo = INT32_MIN;   // "no X entry before the chunk"
-constexpr int32_t kGuardHi = INT32_MAX;   // "no Y entry after the chunk"
+constexpr int kSub = 256;                 // merged positions per LDS-staged sub-step
+constexpr int kPerLane = kSub / kWave;    // merged positions per lane per sub-step
+constexpr int32_t kGuardLo = INT32_MIN;   // "no X entry before the tile"
+constexpr int32_t kGuardHi = INT32_MAX;   // "no Y entry after the row"
 
-// One wave per chunk of kChunk merged positions of a row.  The chunk's ends
-// are split once in global memory; its X and Y ranges (plus the X entry
-// before it and the Y entry after it, which decide folding at the edges) are
-// staged in LDS with coalesced loads, and each lane merges kPerLane
-// consecutive positions from LDS.  A Y entry equal to the preceding X entry
-// is folded into it (output x + y at the X position).  The chunk count is
-// read on the device (cstart[rows]); grid-stride so the launch can be
+// One wave per chunk of kChunk merged positions of a row.  The chunk start
+// is split once in global memory (merge path, ties to X); the wave then walks
+// the chunk in sub-steps of kSub positions.  Each sub-step stages the next
+// kSub entries of X and Y (plus the X entry before them and the Y entry after
+// them, which decide folding at the edges) in LDS with coalesced loads, every
+// lane splits its own kPerLane positions inside the staged tile and merges
+// them, and the wave advances by the X entries it consumed (a wave sum), so
+// no further global search is needed.  A Y entry equal to the preceding X
+// entry is folded into it (output x + y at the X position).  The chunk count
+// is read on the device (cstart[rows]); grid-stride so the launch can be
 // replayed from a graph.
 template <bool kWrite>
 __global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__ tasks, int nt, int32_t rows,
@@ -108,8 +111,8 @@ __global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__
                                                   const int64_t *__restrict__ coff, int32_t *__restrict__ oi,
                                                   double *__restrict__ ov) {
     constexpr int kVW = kWrite ? 4 : 1;
-    constexpr int kVN = kWrite ? kChunk + 2 : 1;
-    __shared__ int32_t s_xi[4][kChunk + 2], s_yi[4][kChunk + 2];
+    constexpr int kVN = kWrite ? kSub + 2 : 1;
+    __shared__ int32_t s_xi[4][kSub + 2], s_yi[4][kSub + 2];
     __shared__ double s_xv[kVW][kVN], s_yv[kVW][kVN];
     const int l = lane_id(), w = threadIdx.x / kWave;
     int32_t *XI = s_xi[w], *YI = s_yi[w];
@@ -125,78 +128,89 @@ __global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__
         const double *__restrict__ yv = T.yv + s.ys;
         const int32_t t0 = (int32_t)(c - cstart[g]) * kChunk;
         const int32_t t1 = min(t0 + kChunk, s.lx + s.ly);
-        // lanes 0 and 1 split the two ends of the chunk
         int32_t sp = 0;
-        if (l < 2) sp = merge_split(xi, s.lx, yi, s.ly, l == 0 ? t0 : t1);
-        const int32_t i0 = __shfl(sp, 0, kWave), i1 = __shfl(sp, 1, kWave);
-        const int32_t j0 = t0 - i0, j1 = t1 - i1;
-        const int32_t nx = i1 - i0, ny = j1 - j0;
-        // XI[u] = X[i0 - 1 + u] for u in [0, nx + 1); YI[u] = Y[j0 + u] for u in [0, ny + 1)
-        for (int u = l; u <= nx; u += kWave) {
-            const int32_t ii = i0 - 1 + u;
-            XI[u] = ii >= 0 ? xi[ii] : kGuardLo;
-            if (kWrite) XV[u] = ii >= 0 ? xv[ii] : 0.0;
-        }
-        for (int u = l; u <= ny; u += kWave) {
-            const int32_t jj = j0 + u;
-            YI[u] = jj < s.ly ? yi[jj] : kGuardHi;
-            if (kWrite) YV[u] = jj < s.ly ? yv[jj] : 0.0;
-        }
-        wave_sync();
-        // this lane's first local position and its split inside the chunk
-        const int32_t d = l * kPerLane, n = nx + ny;
-        int32_t i = 0, j = 0;
-        if (d < n) {
-            int32_t lo = d - ny > 0 ? d - ny : 0, hi = d < nx ? d : nx;
-            while (lo < hi) {
-                const int32_t mid = (lo + hi) >> 1;
-                if (XI[1 + mid] <= YI[d - 1 - mid]) lo = mid + 1;
-                else hi = mid;
+        if (l == 0) sp = merge_split(xi, s.lx, yi, s.ly, t0);
+        int32_t i0 = __shfl(sp, 0, kWave);
+        int32_t j0 = t0 - i0;
+        int64_t out = kWrite ? coff[c] : 0;
+        int32_t kept_total = 0;
+        for (int32_t tb = t0; tb < t1; tb += kSub) {
+            const int32_t n = min(kSub, t1 - tb);
+            const int32_t nxa = min(kSub, s.lx - i0), nya = min(kSub, s.ly - j0);
+            // XI[u] = X[i0 - 1 + u] for u in [0, nxa]; YI[u] = Y[j0 + u] for u in [0, nya]
+            for (int u = l; u <= nxa; u += kWave) {
+                const int32_t ii = i0 - 1 + u;
+                XI[u] = ii >= 0 ? xi[ii] : kGuardLo;
+                if (kWrite) XV[u] = ii >= 0 ? xv[ii] : 0.0;
             }
-            i = lo;
-            j = d - lo;
-        }
-        int32_t keys[kPerLane];
-        double vals[kPerLane];
-        uint32_t keep = 0;
-#pragma unroll
-        for (int u = 0; u < kPerLane; ++u) {
-            keys[u] = 0;
-            vals[u] = 0.0;
-            if (d + u >= n) continue;
-            const bool fromX = (i < nx) && (j >= ny || XI[1 + i] <= YI[j]);
-            if (fromX) {
-                keys[u] = XI[1 + i];
-                if (kWrite) {
-                    vals[u] = XV[1 + i];
-                    if (YI[j] == keys[u]) vals[u] = vals[u] + YV[j];
+            for (int u = l; u <= nya; u += kWave) {
+                const int32_t jj = j0 + u;
+                YI[u] = jj < s.ly ? yi[jj] : kGuardHi;
+                if (kWrite) YV[u] = jj < s.ly ? yv[jj] : 0.0;
+            }
+            wave_sync();
+            // this lane's first position and its split inside the staged tile
+            const int32_t d = l * kPerLane;
+            int32_t i = 0, j = 0;
+            if (d < n) {
+                int32_t lo = d - nya > 0 ? d - nya : 0, hi = d < nxa ? d : nxa;
+                while (lo < hi) {
+                    const int32_t mid = (lo + hi) >> 1;
+                    if (XI[1 + mid] <= YI[d - 1 - mid]) lo = mid + 1;
+                    else hi = mid;
                 }
-                keep |= 1u << u;
-                ++i;
-            } else {
-                keys[u] = YI[j];
-                if (XI[i] != keys[u]) {  // else folded into the preceding X entry
-                    if (kWrite) vals[u] = YV[j];
+                i = lo;
+                j = d - lo;
+            }
+            int32_t keys[kPerLane];
+            double vals[kPerLane];
+            uint32_t keep = 0;
+            int32_t xcnt = 0;
+#pragma unroll
+            for (int u = 0; u < kPerLane; ++u) {
+                keys[u] = 0;
+                vals[u] = 0.0;
+                if (d + u >= n) continue;
+                const bool fromX = (i < nxa) && (j >= nya || XI[1 + i] <= YI[j]);
+                if (fromX) {
+                    keys[u] = XI[1 + i];
+                    if (kWrite) {
+                        vals[u] = XV[1 + i];
+                        if (YI[j] == keys[u]) vals[u] = vals[u] + YV[j];
+                    }
                     keep |= 1u << u;
+                    ++i;
+                    ++xcnt;
+                } else {
+                    keys[u] = YI[j];
+                    if (XI[i] != keys[u]) {  // else folded into the preceding X entry
+                        if (kWrite) vals[u] = YV[j];
+                        keep |= 1u << u;
+                    }
+                    ++j;
                 }
-                ++j;
             }
-        }
-        const int nkeep = __popc(keep);
-        const int32_t incl = wave_incl_scan((int32_t)nkeep);
-        if (kWrite) {
-            int64_t q = coff[c] + (incl - nkeep);
+            const int nkeep = __popc(keep);
+            const int32_t incl = wave_incl_scan((int32_t)nkeep);
+            if (kWrite) {
+                int64_t q = out + (incl - nkeep);
 #pragma unroll
-            for (int u = 0; u < kPerLane; ++u)
-                if (keep & (1u << u)) {
-                    oi[q] = keys[u];
-                    ov[q] = vals[u];
-                    ++q;
-                }
-        } else if (l == kWave - 1) {
-            ccount[c] = incl;
+                for (int u = 0; u < kPerLane; ++u)
+                    if (keep & (1u << u)) {
+                        oi[q] = keys[u];
+                        ov[q] = vals[u];
+                        ++q;
+                    }
+            }
+            const int32_t kept = __shfl(incl, kWave - 1, kWave);
+            out += kept;
+            kept_total += kept;
+            const int32_t xs = wave_sum(xcnt);
+            i0 += xs;
+            j0 += n - xs;
+            wave_sync();  // XI/YI are rewritten by the next sub-step
         }
-        wave_sync();  // XI/YI are rewritten for the next chunk
+        if (!kWrite && l == 0) ccount[c] = kept_total;
     }
 }
 

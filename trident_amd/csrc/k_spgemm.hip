@@ -473,9 +473,10 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
     }
 }
 
-// per item: cursor slots it needs (multi-window items only)
+// per item: cursor slots it needs (multi-window items of rows longer than
+// reg_max entries; shorter rows keep their cursors in registers)
 __global__ void k_item_cursor_len(const Item *__restrict__ items, const Counters *__restrict__ cnt,
-                                  const TaskDev *__restrict__ tasks, int nt, int32_t cap,
+                                  const TaskDev *__restrict__ tasks, int nt, int32_t cap, int32_t reg_max,
                                   int32_t *__restrict__ len) {
     const int32_t it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= cap) return;
@@ -486,6 +487,7 @@ __global__ void k_item_cursor_len(const Item *__restrict__ items, const Counters
             const TaskDev &TK = tasks[task_of(tasks, nt, I.row)];
             const int32_t r = I.row - TK.row_base;
             v = TK.ap[r + 1] - TK.ap[r];
+            if (v <= reg_max) v = 0;
         }
     }
     len[it] = v;
@@ -581,6 +583,7 @@ __global__ __launch_bounds__(256) void k_num_light(const TaskDev *__restrict__ t
 // products of one step that meet in a slot are applied in thread order (=
 // A-order) through atomicMin rounds.  A presence bitmap over the window gives
 // every entry its sorted rank (prefix popcounts) for the write-out.
+constexpr int kRegSlots = 4;  // register-held A entries per thread in the heavy numeric kernel
 constexpr uint32_t kSlotFree = 0xffffffffu;
 constexpr int kTagBits = 10;                        // thread ids < 1023
 constexpr uint32_t kNoTag = (1u << kTagBits) - 1u;  // slot key = column offset << kTagBits | tag
@@ -643,12 +646,17 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
     if (tid == 0) L.nocc = 0;
     __syncthreads();
     const int32_t nitems = cnt->items;
-    // Workgroups are dispatched round-robin over the 8 XCDs; each XCD takes a
-    // contiguous run of items, so the items of one row (which read the same
-    // B rows) share that XCD's L2.  gridDim.x is a multiple of 8.
-    const int32_t per_xcd = (int32_t)(gridDim.x >> 3);
-    const int32_t slot = TR_XCD_MAP ? (int32_t)(blockIdx.x & 7) * per_xcd + (int32_t)(blockIdx.x >> 3)
-                                    : (int32_t)blockIdx.x;
+#if TR_XCD_MAP
+    // Workgroups are dispatched round-robin over the 8 XCDs: runs of
+    // kXcdRun consecutive items (mostly windows of one row, which read the
+    // same B rows) go to one XCD and the runs rotate over the XCDs.
+    // gridDim.x is a multiple of 8 * kXcdRun.
+    constexpr int32_t kXcdRun = 64;
+    const int32_t r = (int32_t)(blockIdx.x >> 3), x = (int32_t)(blockIdx.x & 7);
+    const int32_t slot = ((r / kXcdRun) * 8 + x) * kXcdRun + r % kXcdRun;
+#else
+    const int32_t slot = (int32_t)blockIdx.x;
+#endif
     for (int32_t itx = slot; itx < nitems; itx += gridDim.x) {
         const Item I = items[itx];
         const int32_t g = I.row;
@@ -666,15 +674,23 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
         const Panel &P = find_panel(panels, npanels, g);
         int32_t *oi_row = P.idx + (rs - P.nnz_begin);
         double *ov_row = P.val + (rs - P.nnz_begin);
-        const bool short_row = (a1 - a0) <= NT;
-        // short rows: the A entry and B range of thread tid live in registers
-        int32_t rk_bs = 0, rk_be = 0, rk_cur = 0;
-        double rk_a = 0.0;
-        if (short_row && a0 + tid < a1) {
-            const int32_t k = ai[a0 + tid];
-            rk_a = av[a0 + tid];
-            rk_bs = bp[k];
-            rk_be = bp[k + 1];
+        // rows with <= NT * kRegSlots entries: the A entries and B ranges of
+        // thread tid (slots tid, tid + NT, ...) and their cursors live in
+        // registers; longer rows keep cursors in the item's cursor array
+        const bool reg_row = (a1 - a0) <= NT * kRegSlots;
+        int32_t rk_bs[kRegSlots], rk_be[kRegSlots], rk_cur[kRegSlots];
+        double rk_a[kRegSlots];
+#pragma unroll
+        for (int u = 0; u < kRegSlots; ++u) {
+            rk_bs[u] = rk_be[u] = rk_cur[u] = 0;
+            rk_a[u] = 0.0;
+            const int32_t p = a0 + u * NT + tid;
+            if (reg_row && p < a1) {
+                const int32_t k = ai[p];
+                rk_a[u] = av[p];
+                rk_bs[u] = bp[k];
+                rk_be[u] = bp[k + 1];
+            }
         }
         bool have_cursor = false;
         for (int32_t wi = I.wbeg; wi < I.wend; ++wi) {
@@ -687,32 +703,43 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             // and are ranked by counting smaller keys
             const bool dense = (W.c1 - W.c0) <= kWinMaxWidth;
             int nfresh = 0;
-            for (int32_t pb = a0; pb < a1; pb += NT) {
+            for (int32_t pb = a0, slot = 0; pb < a1; pb += NT, ++slot) {
                 // this thread's k-slot: A entry p, B segment [s, e) inside the window
                 const int32_t p = pb + tid;
                 int32_t s = 0, e = 0;
                 double a = 0.0;
                 if (p < a1) {
-                    int32_t bs, be;
-                    if (short_row) {
-                        bs = rk_bs;
-                        be = rk_be;
-                        a = rk_a;
+                    int32_t bs = 0, be = 0, c = 0;
+                    if (reg_row) {
+                        // uniform slot index: selects, not dynamic register indexing
+#pragma unroll
+                        for (int u = 0; u < kRegSlots; ++u)
+                            if (u == slot) {
+                                bs = rk_bs[u];
+                                be = rk_be[u];
+                                a = rk_a[u];
+                                c = rk_cur[u];
+                            }
                     } else {
                         const int32_t k = ai[p];
                         a = av[p];
                         bs = bp[k];
                         be = bp[k + 1];
+                        if (have_cursor && !full) c = cur[p - a0];
                     }
                     if (full) {
                         s = bs;
                         e = be;
                     } else {
-                        const int32_t c = short_row ? rk_cur : (have_cursor ? cur[p - a0] : 0);
                         s = have_cursor ? c : lower_bound_dev(bi, bs, be, W.c0);
                         e = gallop_dev(bi, s, be, W.c1);
-                        if (short_row) rk_cur = e;
-                        else if (!last) cur[p - a0] = e;
+                        if (reg_row) {
+#pragma unroll
+                            for (int u = 0; u < kRegSlots; ++u)
+                                if (u == slot) rk_cur[u] = e;
+                        } else if (!last) {
+                            cur[p - a0] = e;
+                        }
                     }
                 }
                 int32_t total;
@@ -909,7 +936,8 @@ uint32_t cap_grid(int64_t need, int64_t cap) {
     return (uint32_t)std::min<int64_t>(need, cap);
 }
 
-uint32_t round_up8(uint32_t g) { return (g + 7u) & ~7u; }
+// heavy numeric grid: a multiple of the XCD run pattern when items are mapped to XCDs
+uint32_t heavy_grid(uint32_t g) { return TR_XCD_MAP ? (g + 511u) & ~511u : g; }
 
 template <int T>
 void launch_sym_light(tr_ctx *ctx, MulPlan &pl, int c) {
@@ -1001,7 +1029,7 @@ int64_t mul_cursor_prep(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool capture) {
     }
     if (ws.owned && !capture) pl.scan_tmp = (int64_t *)ws.get(S_SCAN_TMP, scan_tmp_bytes(std::max<int64_t>(pl.item_cap, pl.rows_total)));
     TR_LAUNCH(ctx, "item_cursor_len", k_item_cursor_len, div_up((uint64_t)pl.item_cap, 256), 256, 0, pl.items, pl.cnt,
-              pl.d_tasks, pl.nt, pl.item_cap, pl.clen);
+              pl.d_tasks, pl.nt, pl.item_cap, (pl.tune.heavy_table == 4096 ? 512 : 256) * kRegSlots, pl.clen);
     launch_exclusive_scan_i32_to_i64(ctx, pl.clen, pl.cbase, pl.item_cap, pl.scan_tmp);
     if (capture) return pl.cursor_cap;
     return read_scalar<int64_t>(ctx, pl.cbase + pl.item_cap);
@@ -1019,7 +1047,7 @@ void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *dp, int np) {
     if (pl.h.cls[4]) launch_num_light<1024>(ctx, pl, 4, dp, np);
     if (pl.h.cls[5]) launch_num_light<2048>(ctx, pl, 5, dp, np);
     if (pl.h.cls[kHeavyClass] && pl.h.items > 0) {
-        const uint32_t grid = round_up8(cap_grid(pl.h.items, 1 << 24));
+        const uint32_t grid = heavy_grid(cap_grid(pl.h.items, 1 << 24));
         if (pl.tune.heavy_table == 4096)
             TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<4096, 512>), grid, 512, 0, pl.d_tasks, pl.nt, pl.items,
                       pl.cnt, pl.wins, pl.cbase, pl.cursors, pl.rowptr, dp, np);

@@ -6,7 +6,7 @@
 #include <vector>
 
 #ifndef TR_XCD_MAP
-#define TR_XCD_MAP 1
+#define TR_XCD_MAP 0
 #endif
 
 namespace tr {
