@@ -64,6 +64,26 @@ def apply_tuning(args):
     return t, os.path.relpath(path, ROOT)
 
 
+def load_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 --pmc summary (profiles/pmc_traffic.txt): FETCH_SIZE and
+    WRITE_SIZE in KiB per dispatch, each scaled by what the same counter read
+    for a plain copy of known size.  None when no measurement is recorded."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if not line.startswith("traffic "):
+                continue
+            kv = dict(tok.split("=", 1) for tok in line.split()[1:])
+            if kv.get("kernel") == kernel and kv.get("workload") == workload:
+                b = 1024.0 * (float(kv["fetch_kib"]) * float(kv["fetch_scale"]) +
+                              float(kv["write_kib"]) * float(kv["write_scale"]))
+                return b, "profiles/pmc_traffic.txt"
+    return None, None
+
+
 def row_flops(A):
     rl = np.diff(A.ptr).astype(np.int64)
     f = np.zeros(A.nrows, np.int64)
@@ -547,6 +567,7 @@ def main():
         alg = 12 * nnzC + 2 * B_bytes
         alg_desc = "whole product: A and B read once, C written once"
     per_launch_s = dms / 1000.0 / max(dlaunch, 1)
+    traffic, traffic_src = load_traffic(dname, args.workload)
     achieved = alg / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
     bytes_alg_total = 12 * nnzC + 4 * (A.nrows + 1) + 2 * B_bytes
     roofline = {
@@ -556,7 +577,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": None if traffic is None else int(traffic),
+        "traffic_source": traffic_src,
         "alg_bytes_per_launch": int(alg),
         "alg_bytes_def": alg_desc,
         "kernel_ms_per_launch": round(1000 * per_launch_s, 3),

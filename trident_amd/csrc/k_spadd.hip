@@ -90,6 +90,7 @@ __device__ __forceinline__ int32_t merge_split(const int32_t *__restrict__ xi, i
 
 constexpr int kSub = 256;                 // merged positions per LDS-staged sub-step
 constexpr int kPerLane = kSub / kWave;    // merged positions per lane per sub-step
+constexpr int kStage = (kSub + 1) / kWave + 1;  // tile entries (kSub + 1) per lane, rounded up
 constexpr int32_t kGuardLo = INT32_MIN;   // "no X entry before the tile"
 constexpr int32_t kGuardHi = INT32_MAX;   // "no Y entry after the row"
 
@@ -137,16 +138,37 @@ __global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__
         for (int32_t tb = t0; tb < t1; tb += kSub) {
             const int32_t n = min(kSub, t1 - tb);
             const int32_t nxa = min(kSub, s.lx - i0), nya = min(kSub, s.ly - j0);
-            // XI[u] = X[i0 - 1 + u] for u in [0, nxa]; YI[u] = Y[j0 + u] for u in [0, nya]
-            for (int u = l; u <= nxa; u += kWave) {
-                const int32_t ii = i0 - 1 + u;
-                XI[u] = ii >= 0 ? xi[ii] : kGuardLo;
-                if (kWrite) XV[u] = ii >= 0 ? xv[ii] : 0.0;
+            // XI[u] = X[i0 - 1 + u] for u in [0, nxa]; YI[u] = Y[j0 + u] for u in [0, nya];
+            // all loads of the tile are issued before the first LDS store
+            int32_t rxi[kStage], ryi[kStage];
+            double rxv[kStage], ryv[kStage];
+#pragma unroll
+            for (int k = 0; k < kStage; ++k) {
+                const int u = l + k * kWave;
+                const int32_t ii = i0 - 1 + u, jj = j0 + u;
+                rxi[k] = kGuardLo;
+                ryi[k] = kGuardHi;
+                rxv[k] = ryv[k] = 0.0;
+                if (u <= nxa && ii >= 0) {
+                    rxi[k] = xi[ii];
+                    if (kWrite) rxv[k] = xv[ii];
+                }
+                if (u <= nya && jj < s.ly) {
+                    ryi[k] = yi[jj];
+                    if (kWrite) ryv[k] = yv[jj];
+                }
             }
-            for (int u = l; u <= nya; u += kWave) {
-                const int32_t jj = j0 + u;
-                YI[u] = jj < s.ly ? yi[jj] : kGuardHi;
-                if (kWrite) YV[u] = jj < s.ly ? yv[jj] : 0.0;
+#pragma unroll
+            for (int k = 0; k < kStage; ++k) {
+                const int u = l + k * kWave;
+                if (u <= nxa) {
+                    XI[u] = rxi[k];
+                    if (kWrite) XV[u] = rxv[k];
+                }
+                if (u <= nya) {
+                    YI[u] = ryi[k];
+                    if (kWrite) YV[u] = ryv[k];
+                }
             }
             wave_sync();
             // this lane's first position and its split inside the staged tile

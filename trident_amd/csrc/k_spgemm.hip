@@ -500,19 +500,24 @@ __global__ void k_panel_bases(Panel *__restrict__ panels, int np, const int64_t 
 }
 
 // ---------------------------------------------------------------- numeric, light rows
-template <int T>
-__global__ __launch_bounds__(256) void k_num_light(const TaskDev *__restrict__ tasks, int nt,
-                                                   const int32_t *__restrict__ rows, Counters *__restrict__ cnt,
-                                                   int cls, const int64_t *__restrict__ rowptr,
-                                                   const Panel *__restrict__ panels, int npanels) {
-    __shared__ int32_t keys_s[4][T];
-    __shared__ double vals_s[4][T];
+constexpr int kLightPre = 4;  // B rows prefetched together per batch of a light row
+
+// WPB waves per block, one row per wave (big tables use one-wave blocks so
+// that LDS, not the block shape, bounds the waves per CU)
+template <int T, int WPB>
+__global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__restrict__ tasks, int nt,
+                                                           const int32_t *__restrict__ rows,
+                                                           Counters *__restrict__ cnt, int cls,
+                                                           const int64_t *__restrict__ rowptr,
+                                                           const Panel *__restrict__ panels, int npanels) {
+    __shared__ int32_t keys_s[WPB][T];
+    __shared__ double vals_s[WPB][T];
     constexpr uint32_t kLog = __builtin_ctz(T);
     const int w = threadIdx.x / kWave, l = lane_id();
     int32_t *keys = keys_s[w];
     double *vals = vals_s[w];
     const int32_t nrows = cnt->cls[cls];
-    for (int32_t li = blockIdx.x * 4 + w; li < nrows; li += gridDim.x * 4) {
+    for (int32_t li = blockIdx.x * WPB + w; li < nrows; li += gridDim.x * WPB) {
         const int32_t g = rows[li];
         const TaskDev &TK = tasks[task_of(tasks, nt, g)];
         const int32_t r = g - TK.row_base;
@@ -534,24 +539,49 @@ __global__ __launch_bounds__(256) void k_num_light(const TaskDev *__restrict__ t
                 be = bp[k + 1];
             }
             const int nk = min(kWave, a1 - pb);
-            for (int t = 0; t < nk; ++t) {
+            auto ins = [&](int32_t j, double b, double at) {
+                uint32_t h = hash_slot(j, kLog);
+                for (int probe = 0; probe < T; ++probe) {
+                    const int32_t old = atomicCAS(&keys[h], kEmpty, j);
+                    if (old == kEmpty) {
+                        vals[h] = __fma_rn(at, b, 0.0);
+                        break;
+                    }
+                    if (old == j) {
+                        vals[h] = __fma_rn(at, b, vals[h]);
+                        break;
+                    }
+                    h = (h + 1) & (T - 1);
+                }
+            };
+            // the first 64 entries of the first kLightPre B rows are loaded
+            // together (one round trip instead of one per row), then applied
+            // row by row in A-order
+            int32_t pj[kLightPre];
+            double pv[kLightPre];
+#pragma unroll
+            for (int t = 0; t < kLightPre; ++t) {
+                pj[t] = 0;
+                pv[t] = 0.0;
+                const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
+                if (t < nk && s + l < e) {
+                    pj[t] = bi[s + l];
+                    pv[t] = bv[s + l];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < kLightPre; ++t) {
+                if (t < nk) {
+                    const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
+                    const double at = __shfl(a, t, kWave);
+                    if (s + l < e) ins(pj[t], pv[t], at);
+                    stream_entries(bi, bv, s + kWave, e, [&](int32_t j, double b) { ins(j, b, at); });
+                }
+            }
+            for (int t = kLightPre; t < nk; ++t) {
                 const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
                 const double at = __shfl(a, t, kWave);
-                stream_entries(bi, bv, s, e, [&](int32_t j, double b) {
-                    uint32_t h = hash_slot(j, kLog);
-                    for (int probe = 0; probe < T; ++probe) {
-                        const int32_t old = atomicCAS(&keys[h], kEmpty, j);
-                        if (old == kEmpty) {
-                            vals[h] = __fma_rn(at, b, 0.0);
-                            break;
-                        }
-                        if (old == j) {
-                            vals[h] = __fma_rn(at, b, vals[h]);
-                            break;
-                        }
-                        h = (h + 1) & (T - 1);
-                    }
-                });
+                stream_entries(bi, bv, s, e, [&](int32_t j, double b) { ins(j, b, at); });
             }
         }
         wave_sync();
@@ -826,6 +856,11 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             int32_t *oi = oi_row + W.off;
             double *ov = ov_row + W.off;
             const int nwords = dense ? (W.c1 - W.c0 + 63) >> 6 : kBmWords;
+            // each thread's entries: rank, column, value (rank < 0: none)
+            constexpr int kSt = T / NT;
+            static_assert(kSt * NT >= kSparseH + 127, "a sparse window must fit the per-thread stash");
+            int32_t st_r[kSt], st_k[kSt];
+            double st_v[kSt];
             if (dense) {
                 // prefix popcounts of the presence words
                 constexpr int kPer = kBmWords / NT > 0 ? kBmWords / NT : 1;
@@ -847,16 +882,17 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                 }
                 __syncthreads();
                 if (tid == 0 && L.nocc != W.cnt) atomicOr(&cnt->overflow, 4);
-                for (int h = tid; h < T; h += NT) {
+#pragma unroll
+                for (int u = 0; u < kSt; ++u) {
+                    const int h = tid + u * NT;
                     const uint32_t kt = L.keytag[h];
+                    st_r[u] = -1;
                     if (kt == kSlotFree) continue;
                     const uint32_t off = kt >> kTagBits;
                     const int wd = (int)(off >> 6);
-                    const int32_t rr = (int32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
-                    if (rr < W.cnt) {  // never write past the window
-                        oi[rr] = W.c0 + (int32_t)off;
-                        ov[rr] = L.vals[h];
-                    }
+                    st_r[u] = (int32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
+                    st_k[u] = W.c0 + (int32_t)off;
+                    st_v[u] = L.vals[h];
                 }
             } else {
                 // compact the occupied slots (keys into the bitmap's space,
@@ -879,16 +915,33 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                     }
                 __syncthreads();
                 if (tid == 0 && (L.nocc != W.cnt || !fits)) atomicOr(&cnt->overflow, 4);
-                if (fits)
-                    for (int e = tid; e < tot; e += NT) {
-                        const uint32_t k = lkey[e];
-                        int32_t rr = 0;
-                        for (int u = 0; u < tot; ++u) rr += lkey[u] < k ? 1 : 0;
-                        if (rr < W.cnt) {
-                            oi[rr] = W.c0 + (int32_t)k;
-                            ov[rr] = L.vals[lslot[e]];
-                        }
-                    }
+#pragma unroll
+                for (int u = 0; u < kSt; ++u) {
+                    const int e = tid + u * NT;
+                    st_r[u] = -1;
+                    if (!fits || e >= tot) continue;
+                    const uint32_t k = lkey[e];
+                    int32_t rr = 0;
+                    for (int q = 0; q < tot; ++q) rr += lkey[q] < k ? 1 : 0;
+                    st_r[u] = rr;
+                    st_k[u] = W.c0 + (int32_t)k;
+                    st_v[u] = L.vals[lslot[e]];
+                }
+            }
+            // stage the window's entries in rank order in the table's space,
+            // then write them out with coalesced stores
+            __syncthreads();
+            int32_t *sidx = reinterpret_cast<int32_t *>(L.keytag);
+#pragma unroll
+            for (int u = 0; u < kSt; ++u)
+                if (st_r[u] >= 0 && st_r[u] < W.cnt) {  // never write past the window
+                    sidx[st_r[u]] = st_k[u];
+                    L.vals[st_r[u]] = st_v[u];
+                }
+            __syncthreads();
+            for (int e = tid; e < W.cnt; e += NT) {
+                oi[e] = sidx[e];
+                ov[e] = L.vals[e];
             }
             __syncthreads();
             for (int t = tid; t < T; t += NT) L.keytag[t] = kSlotFree;
@@ -948,9 +1001,10 @@ void launch_sym_light(tr_ctx *ctx, MulPlan &pl, int c) {
 
 template <int T>
 void launch_num_light(tr_ctx *ctx, MulPlan &pl, int c, Panel *panels, int np) {
+    constexpr int kWpb = T >= 1024 ? 1 : 4;
     const int32_t n = pl.h.cls[c];
-    TR_LAUNCH(ctx, "num_light", k_num_light<T>, cap_grid(div_up((uint64_t)n, 4), 1 << 20), 256, 0, pl.d_tasks,
-              pl.nt, pl.lists + (int64_t)c * pl.rows_total, pl.cnt, c, pl.rowptr, panels, np);
+    TR_LAUNCH(ctx, "num_light", (k_num_light<T, kWpb>), cap_grid(div_up((uint64_t)n, kWpb), 1 << 20), kWpb * kWave,
+              0, pl.d_tasks, pl.nt, pl.lists + (int64_t)c * pl.rows_total, pl.cnt, c, pl.rowptr, panels, np);
 }
 
 }  // namespace
