@@ -54,26 +54,37 @@ __device__ __forceinline__ int32_t lower_bound_dev(const int32_t *__restrict__ a
     return lo;
 }
 
-// first q >= s with a[q] >= key, galloping from s (cheap when the answer is
-// close to s, as for consecutive windows)
-__device__ __forceinline__ int32_t gallop_dev(const int32_t *__restrict__ a, int32_t s, int32_t e, int32_t key) {
-    if (s >= e || a[s] >= key) return s;
-    int32_t lo = s, step = 1, hi;
+// first q in (lo, e] with a[q] >= key (q == e if none), galloping from lo,
+// where a[lo] < key is known; vq receives a[q] (INT32_MAX when q == e), the
+// column the next window of a heavy row starts from
+__device__ __forceinline__ int32_t gallop_from(const int32_t *__restrict__ a, int32_t lo, int32_t e, int32_t key,
+                                               int32_t &vq) {
+    int32_t step = 1, hi;
+    vq = INT32_MAX;
     while (true) {
         hi = lo + step;
         if (hi >= e) {
             hi = e;
+            vq = INT32_MAX;
             break;
         }
-        if (a[hi] >= key) break;
+        const int32_t v = a[hi];
+        if (v >= key) {
+            vq = v;
+            break;
+        }
         lo = hi;
         step <<= 1;
     }
-    // a[lo] < key, (hi == e || a[hi] >= key)
     while (hi - lo > 1) {
-        int32_t mid = lo + ((hi - lo) >> 1);
-        if (a[mid] < key) lo = mid;
-        else hi = mid;
+        const int32_t mid = lo + ((hi - lo) >> 1);
+        const int32_t v = a[mid];
+        if (v < key) {
+            lo = mid;
+        } else {
+            hi = mid;
+            vq = v;
+        }
     }
     return hi;
 }
@@ -628,8 +639,9 @@ struct HeavyWgLds {
     int32_t seg_x[NT + 1];
     double seg_a[NT];
     int32_t wsum[NT / kWave];
-    int32_t flag[2];
     int32_t nocc;
+    double pa[NT], pb[NT];  // the step's products (A value, B value) by thread
+    uint16_t link[NT];      // next older product of the same slot in this step
 };
 
 // workgroup exclusive scan of v; returns the exclusive prefix, *total = sum
@@ -666,7 +678,7 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                                                      const Item *__restrict__ items, Counters *__restrict__ cnt,
                                                      const Window *__restrict__ wins,
                                                      const int64_t *__restrict__ cursor_base,
-                                                     int32_t *__restrict__ cursors, const int64_t *__restrict__ rowptr,
+                                                     int2 *__restrict__ cursors, const int64_t *__restrict__ rowptr,
                                                      const Panel *__restrict__ panels, int npanels) {
     constexpr uint32_t kLog = __builtin_ctz(T);
     __shared__ HeavyWgLds<T, NT> L;
@@ -699,7 +711,7 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
         const double *__restrict__ bv = TK.bv;
         const int32_t ncols = TK.n;
         const int32_t a0 = TK.ap[r], a1 = TK.ap[r + 1];
-        int32_t *cur = cursors + cursor_base[itx];
+        int2 *cur = cursors + cursor_base[itx];
         const int64_t rs = rowptr[g];
         const Panel &P = find_panel(panels, npanels, g);
         int32_t *oi_row = P.idx + (rs - P.nnz_begin);
@@ -708,17 +720,18 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
         // thread tid (slots tid, tid + NT, ...) and their cursors live in
         // registers; longer rows keep cursors in the item's cursor array
         const bool reg_row = (a1 - a0) <= NT * kRegSlots;
-        int32_t rk_bs[kRegSlots], rk_be[kRegSlots], rk_cur[kRegSlots];
+        // (before the item's first window the cursor holds the start of B_k)
+        int32_t rk_be[kRegSlots], rk_cur[kRegSlots], rk_nxt[kRegSlots];
         double rk_a[kRegSlots];
 #pragma unroll
         for (int u = 0; u < kRegSlots; ++u) {
-            rk_bs[u] = rk_be[u] = rk_cur[u] = 0;
+            rk_be[u] = rk_cur[u] = rk_nxt[u] = 0;
             rk_a[u] = 0.0;
             const int32_t p = a0 + u * NT + tid;
             if (reg_row && p < a1) {
                 const int32_t k = ai[p];
                 rk_a[u] = av[p];
-                rk_bs[u] = bp[k];
+                rk_cur[u] = bp[k];
                 rk_be[u] = bp[k + 1];
             }
         }
@@ -739,36 +752,52 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                 int32_t s = 0, e = 0;
                 double a = 0.0;
                 if (p < a1) {
-                    int32_t bs = 0, be = 0, c = 0;
+                    // c: cursor (first entry at or after this window's start),
+                    // v: its column (INT32_MAX at the end of B_k); known once the
+                    // item's first window has been visited
+                    int32_t bs = 0, be = 0, c = 0, v = 0;
                     if (reg_row) {
                         // uniform slot index: selects, not dynamic register indexing
 #pragma unroll
                         for (int u = 0; u < kRegSlots; ++u)
                             if (u == slot) {
-                                bs = rk_bs[u];
                                 be = rk_be[u];
                                 a = rk_a[u];
                                 c = rk_cur[u];
+                                v = rk_nxt[u];
                             }
+                        bs = c;
                     } else {
                         const int32_t k = ai[p];
                         a = av[p];
                         bs = bp[k];
                         be = bp[k + 1];
-                        if (have_cursor && !full) c = cur[p - a0];
+                        if (have_cursor && !full) {
+                            const int2 cv = cur[p - a0];
+                            c = cv.x;
+                            v = cv.y;
+                        }
                     }
                     if (full) {
                         s = bs;
                         e = be;
                     } else {
-                        s = have_cursor ? c : lower_bound_dev(bi, bs, be, W.c0);
-                        e = gallop_dev(bi, s, be, W.c1);
+                        if (!have_cursor) {
+                            c = lower_bound_dev(bi, bs, be, W.c0);
+                            v = c < be ? bi[c] : INT32_MAX;
+                        }
+                        s = c;
+                        // an A entry with nothing in this window costs no load
+                        e = v >= W.c1 ? s : gallop_from(bi, s, be, W.c1, v);
                         if (reg_row) {
 #pragma unroll
                             for (int u = 0; u < kRegSlots; ++u)
-                                if (u == slot) rk_cur[u] = e;
+                                if (u == slot) {
+                                    rk_cur[u] = e;
+                                    rk_nxt[u] = v;
+                                }
                         } else if (!last) {
-                            cur[p - a0] = e;
+                            cur[p - a0] = make_int2(e, v);
                         }
                     }
                 }
@@ -803,14 +832,19 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                         jn = bi[q];
                         bn = bv[q];
                     }
+                    // insert: the first product of a slot in this step creates or
+                    // claims it; every later one pushes itself onto the slot's
+                    // list (the tag field holds the last pusher, link[] the rest)
                     uint32_t h = 0, key = 0;
                     if (act) {
                         const uint32_t off = (uint32_t)(j - W.c0);
                         key = off << kTagBits;
+                        const uint32_t me = key | (uint32_t)tid;
                         h = hash_slot((int32_t)off, kLog);
+                        uint32_t prev = kNoTag;
                         bool placed = false;
                         for (int probe = 0; probe < T; ++probe) {
-                            const uint32_t old = atomicCAS(&L.keytag[h], kSlotFree, key | kNoTag);
+                            const uint32_t old = atomicCAS(&L.keytag[h], kSlotFree, me);
                             if (old == kSlotFree) {
                                 L.vals[h] = 0.0;
                                 if (dense) atomicOr(&L.bm[off >> 6], 1ull << (off & 63));
@@ -819,6 +853,7 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                                 break;
                             }
                             if ((old >> kTagBits) == off) {
+                                prev = atomicExch(&L.keytag[h], me) & kNoTag;
                                 placed = true;
                                 break;
                             }
@@ -827,25 +862,39 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                         if (!placed) {
                             atomicOr(&cnt->overflow, 8);  // table full: symbolic counts were wrong
                             act = false;
+                        } else {
+                            L.link[tid] = (uint16_t)prev;
+                            L.pa[tid] = ao;
+                            L.pb[tid] = b;
                         }
                     }
-                    if (tid == 0) L.flag[0] = 0;
                     __syncthreads();
-                    // apply in thread order: the lowest pending thread of each slot per round
-                    bool pending = act;
-                    for (int round = 0;; ++round) {
-                        if (pending) atomicMin(&L.keytag[h], key | (uint32_t)tid);
-                        __syncthreads();
-                        if (pending && (L.keytag[h] & kNoTag) == (uint32_t)tid) {
-                            L.vals[h] = __fma_rn(ao, b, L.vals[h]);
-                            L.keytag[h] = key | kNoTag;
-                            pending = false;
+                    // the last pusher of each slot applies the slot's products of
+                    // this step in thread (= A) order: the list is gathered into a
+                    // bitmask of thread ids and walked from the lowest bit
+                    if (act && (L.keytag[h] & kNoTag) == (uint32_t)tid) {
+                        uint64_t mk[NT / kWave];
+#pragma unroll
+                        for (int w = 0; w < NT / kWave; ++w) mk[w] = 0ull;
+                        for (uint32_t t = (uint32_t)tid; t != kNoTag; t = L.link[t]) {
+#pragma unroll
+                            for (int w = 0; w < NT / kWave; ++w)
+                                if ((int)(t >> 6) == w) mk[w] |= 1ull << (t & 63);
                         }
-                        if (ballot(pending) && l == 0) L.flag[round & 1] = 1;
-                        if (tid == 0) L.flag[(round + 1) & 1] = 0;
-                        __syncthreads();
-                        if (!L.flag[round & 1]) break;
+                        double v = L.vals[h];
+#pragma unroll
+                        for (int w = 0; w < NT / kWave; ++w) {
+                            uint64_t mw = mk[w];
+                            while (mw) {
+                                const int t = w * kWave + __ffsll((unsigned long long)mw) - 1;
+                                mw &= mw - 1ull;
+                                v = __fma_rn(L.pa[t], L.pb[t], v);
+                            }
+                        }
+                        L.vals[h] = v;
+                        L.keytag[h] = key | kNoTag;
                     }
+                    __syncthreads();
                 }
                 __syncthreads();  // seg_* are rewritten by the next batch
             }
@@ -856,10 +905,12 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             int32_t *oi = oi_row + W.off;
             double *ov = ov_row + W.off;
             const int nwords = dense ? (W.c1 - W.c0 + 63) >> 6 : kBmWords;
-            // each thread's entries: rank, column, value (rank < 0: none)
+            // each thread's entries: rank << 20 | column offset (kNone: no
+            // entry) and value
             constexpr int kSt = T / NT;
             static_assert(kSt * NT >= kSparseH + 127, "a sparse window must fit the per-thread stash");
-            int32_t st_r[kSt], st_k[kSt];
+            constexpr uint32_t kNone = 0xffffffffu;
+            uint32_t st_rk[kSt];
             double st_v[kSt];
             if (dense) {
                 // prefix popcounts of the presence words
@@ -886,12 +937,12 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                 for (int u = 0; u < kSt; ++u) {
                     const int h = tid + u * NT;
                     const uint32_t kt = L.keytag[h];
-                    st_r[u] = -1;
+                    st_rk[u] = kNone;
                     if (kt == kSlotFree) continue;
                     const uint32_t off = kt >> kTagBits;
                     const int wd = (int)(off >> 6);
-                    st_r[u] = (int32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
-                    st_k[u] = W.c0 + (int32_t)off;
+                    const uint32_t rr = (uint32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
+                    st_rk[u] = (rr << 20) | off;
                     st_v[u] = L.vals[h];
                 }
             } else {
@@ -918,13 +969,12 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
 #pragma unroll
                 for (int u = 0; u < kSt; ++u) {
                     const int e = tid + u * NT;
-                    st_r[u] = -1;
+                    st_rk[u] = kNone;
                     if (!fits || e >= tot) continue;
                     const uint32_t k = lkey[e];
-                    int32_t rr = 0;
-                    for (int q = 0; q < tot; ++q) rr += lkey[q] < k ? 1 : 0;
-                    st_r[u] = rr;
-                    st_k[u] = W.c0 + (int32_t)k;
+                    uint32_t rr = 0;
+                    for (int q = 0; q < tot; ++q) rr += lkey[q] < k ? 1u : 0u;
+                    st_rk[u] = (rr << 20) | k;
                     st_v[u] = L.vals[lslot[e]];
                 }
             }
@@ -933,11 +983,13 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             __syncthreads();
             int32_t *sidx = reinterpret_cast<int32_t *>(L.keytag);
 #pragma unroll
-            for (int u = 0; u < kSt; ++u)
-                if (st_r[u] >= 0 && st_r[u] < W.cnt) {  // never write past the window
-                    sidx[st_r[u]] = st_k[u];
-                    L.vals[st_r[u]] = st_v[u];
+            for (int u = 0; u < kSt; ++u) {
+                const int32_t rr = (int32_t)(st_rk[u] >> 20);
+                if (st_rk[u] != kNone && rr < W.cnt) {  // never write past the window
+                    sidx[rr] = W.c0 + (int32_t)(st_rk[u] & 0xfffffu);
+                    L.vals[rr] = st_v[u];
                 }
+            }
             __syncthreads();
             for (int e = tid; e < W.cnt; e += NT) {
                 oi[e] = sidx[e];
@@ -1240,7 +1292,7 @@ int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max
     Panel *dp = (Panel *)ws.get(S_TMP_CSX, sizeof(Panel) * (size_t)np);
     TR_HIP(hipMemcpyAsync(dp, hp.data(), sizeof(Panel) * (size_t)np, hipMemcpyHostToDevice, ctx->stream));
     if (pl.h.cls[kHeavyClass] && pl.h.items > 0)
-        pl.cursors = (int32_t *)ws.get(S_CURSORS, sizeof(int32_t) * (size_t)std::max<int64_t>(ncur, 1));
+        pl.cursors = (int2 *)ws.get(S_CURSORS, sizeof(int2) * (size_t)std::max<int64_t>(ncur, 1));
     mul_numeric(ctx, pl, dp, np);
     mul_check(ctx, pl);
     return TR_OK;

@@ -2,6 +2,8 @@
 // stage executor (k_batch.hip).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <vector>
 
@@ -102,7 +104,7 @@ struct MulPlan {
     int32_t win_cap = 0, item_cap = 0;
     int32_t *clen = nullptr;
     int64_t *cbase = nullptr;
-    int32_t *cursors = nullptr;
+    int2 *cursors = nullptr;  // (cursor, column at the cursor) per A entry of multi-window items
     int64_t cursor_cap = 0;
     int64_t *scan_tmp = nullptr;  // owned scan temp (graph mode)
     size_t scan_tmp_bytes = 0;
