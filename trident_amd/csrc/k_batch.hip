@@ -163,6 +163,7 @@ struct tr_batch {
     std::vector<int64_t> optr_off;   // per task: first row pointer in o_ptr
     std::vector<int64_t> o_nnz;
     int64_t flops = 0;
+    bool no_accum = false;  // no task has an accumulator: results are the products
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool ran = false;
@@ -257,9 +258,6 @@ void batch_body(tr_batch *b, bool capture) {
     mul_numeric(ctx, b->mp, b->d_panel, 1);
     if (!capture) {
         mul_check(ctx, b->mp);
-        // add tasks: X = products (64-bit global rows), Y = C_in
-        std::vector<AddTaskDev> at(b->nt);
-        int64_t ny = 0;
         b->optr_off.assign(b->nt, 0);
         int64_t optr_total = 0;
         for (int t = 0; t < b->nt; ++t) {
@@ -267,6 +265,25 @@ void batch_body(tr_batch *b, bool capture) {
             optr_total += b->A[t].nrows + 1;
         }
         b->o_ptr = (int32_t *)b->own(sizeof(int32_t) * (size_t)std::max<int64_t>(optr_total, 1));
+    }
+    if (b->no_accum) {
+        // no accumulators: the products are the results (per-task int32 row
+        // pointers rebased from the 64-bit product row pointer)
+        for (int t = 0; t < b->nt; ++t) {
+            const int32_t rb = b->mp.h_tasks[t].row_base;
+            launch_rowptr_i64_to_i32(ctx, b->mp.rowptr, rb, rb + b->A[t].nrows, b->o_ptr + b->optr_off[t]);
+        }
+        if (!capture) {
+            b->o_idx = b->p_idx;
+            b->o_val = b->p_val;
+            b->o_total = b->p_total;
+        }
+        return;
+    }
+    if (!capture) {
+        // add tasks: X = products (64-bit global rows), Y = C_in
+        std::vector<AddTaskDev> at(b->nt);
+        int64_t ny = 0;
         for (int t = 0; t < b->nt; ++t) {
             AddTaskDev &a = at[t];
             a = AddTaskDev{};
@@ -354,6 +371,9 @@ int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_cs
             }
             max_m = std::max(max_m, b->A[t].nrows);
         }
+        b->no_accum = true;
+        for (int t = 0; t < ntasks; ++t)
+            if (b->Cin[t].ptr_vec) b->no_accum = false;
         b->zero_ptr = (int32_t *)b->own(sizeof(int32_t) * ((size_t)max_m + 1));
         TR_HIP(hipMemsetAsync(b->zero_ptr, 0, sizeof(int32_t) * ((size_t)max_m + 1), ctx->stream));
         // compression buffers (capacity = input nnz)
@@ -410,6 +430,10 @@ int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_cs
         b->o_base.assign(ntasks, 0);
         b->o_nnz.assign(ntasks, 0);
         for (int t = 0; t < ntasks; ++t) {
+            if (b->no_accum) {
+                b->o_base[t] = read_scalar<int64_t>(ctx, b->mp.rowptr + b->mp.h_tasks[t].row_base);
+                continue;
+            }
             const int32_t g = b->ap.h[t].row_base;
             const int64_t cs = read_scalar<int64_t>(ctx, b->ap.cstart + g);
             b->o_base[t] = read_scalar<int64_t>(ctx, b->ap.coff + cs);

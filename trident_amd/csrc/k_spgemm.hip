@@ -504,6 +504,90 @@ __global__ void k_item_cursor_len(const Item *__restrict__ items, const Counters
     len[it] = v;
 }
 
+// ---------------------------------------------------------------- heavy item order
+// Heavy items are processed in the column order of their first window, so
+// the workgroups resident at one time work in one column region of B and
+// share its lines through L2 (the items of one row still run left to right
+// within their item).  Counting sort by column bucket: a block-aggregated
+// histogram, a scan, and a scatter with block-local ranks.
+constexpr int kOrderShift = 12;     // bucket = first window's c0 >> kOrderShift
+constexpr int kOrderBuckets = 4096;
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, int32_t n) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+__device__ __forceinline__ int item_bucket(const Item *__restrict__ items, const Window *__restrict__ wins,
+                                           int32_t it) {
+    const int32_t c0 = wins[items[it].wbeg].c0;
+    return min(c0 >> kOrderShift, kOrderBuckets - 1);
+}
+
+__global__ __launch_bounds__(256) void k_item_hist(const Item *__restrict__ items, const Window *__restrict__ wins,
+                                                   const Counters *__restrict__ cnt,
+                                                   unsigned long long *__restrict__ hist) {
+    __shared__ int32_t h[kOrderBuckets];
+    for (int t = threadIdx.x; t < kOrderBuckets; t += 256) h[t] = 0;
+    __syncthreads();
+    const int32_t n = cnt->items;
+    for (int32_t it = blockIdx.x * 256 + threadIdx.x; it < n; it += gridDim.x * 256)
+        atomicAdd(&h[item_bucket(items, wins, it)], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kOrderBuckets; t += 256)
+        if (h[t]) atomicAdd(&hist[t], (unsigned long long)h[t]);
+}
+
+// hist[0..kOrderBuckets) -> exclusive offsets, in place (one block)
+__global__ __launch_bounds__(1024) void k_item_hist_scan(unsigned long long *__restrict__ hist) {
+    __shared__ unsigned long long part[1024 / kWave];
+    constexpr int kPer = kOrderBuckets / 1024;
+    const int tid = threadIdx.x, w = tid / kWave, l = lane_id();
+    unsigned long long v[kPer], sum = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        v[u] = hist[tid * kPer + u];
+        sum += v[u];
+    }
+    const unsigned long long inc = wave_incl_scan(sum);
+    if (l == kWave - 1) part[w] = inc;
+    __syncthreads();
+    unsigned long long off = 0;
+    for (int u = 0; u < w; ++u) off += part[u];
+    unsigned long long run = off + inc - sum;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        hist[tid * kPer + u] = run;
+        run += v[u];
+    }
+}
+
+// The same grid walk as k_item_hist: block-local ranks per bucket, one
+// global cursor bump per (block, bucket)
+__global__ __launch_bounds__(256) void k_item_scatter(const Item *__restrict__ items, const Window *__restrict__ wins,
+                                                      const Counters *__restrict__ cnt,
+                                                      unsigned long long *__restrict__ cursor,
+                                                      int32_t *__restrict__ order) {
+    __shared__ int32_t h[kOrderBuckets];
+    __shared__ unsigned long long base[kOrderBuckets];
+    const int32_t n = cnt->items;
+    for (int32_t it0 = blockIdx.x * 256; it0 < n; it0 += gridDim.x * 256) {
+        for (int t = threadIdx.x; t < kOrderBuckets; t += 256) h[t] = 0;
+        __syncthreads();
+        const int32_t it = it0 + threadIdx.x;
+        int b = 0, r = 0;
+        if (it < n) {
+            b = item_bucket(items, wins, it);
+            r = atomicAdd(&h[b], 1);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < kOrderBuckets; t += 256)
+            if (h[t]) base[t] = atomicAdd(&cursor[t], (unsigned long long)h[t]);
+        __syncthreads();
+        if (it < n) order[base[b] + r] = it;
+        __syncthreads();
+    }
+}
+
 // panel p starts at global row row_begin: its first entry's global offset
 __global__ void k_panel_bases(Panel *__restrict__ panels, int np, const int64_t *__restrict__ rowptr) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -677,6 +761,7 @@ template <int T, int NT>
 __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__ tasks, int nt,
                                                      const Item *__restrict__ items, Counters *__restrict__ cnt,
                                                      const Window *__restrict__ wins,
+                                                     const int32_t *__restrict__ order,
                                                      const int64_t *__restrict__ cursor_base,
                                                      int2 *__restrict__ cursors, const int64_t *__restrict__ rowptr,
                                                      const Panel *__restrict__ panels, int npanels) {
@@ -699,7 +784,8 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
 #else
     const int32_t slot = (int32_t)blockIdx.x;
 #endif
-    for (int32_t itx = slot; itx < nitems; itx += gridDim.x) {
+    for (int32_t ord = slot; ord < nitems; ord += gridDim.x) {
+        const int32_t itx = order[ord];
         const Item I = items[itx];
         const int32_t g = I.row;
         const TaskDev &TK = tasks[task_of(tasks, nt, g)];
@@ -1136,6 +1222,17 @@ int64_t mul_cursor_prep(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool capture) {
     if (ws.owned && !capture) pl.scan_tmp = (int64_t *)ws.get(S_SCAN_TMP, scan_tmp_bytes(std::max<int64_t>(pl.item_cap, pl.rows_total)));
     TR_LAUNCH(ctx, "item_cursor_len", k_item_cursor_len, div_up((uint64_t)pl.item_cap, 256), 256, 0, pl.items, pl.cnt,
               pl.d_tasks, pl.nt, pl.item_cap, (pl.tune.heavy_table == 4096 ? 512 : 256) * kRegSlots, pl.clen);
+    // column order of the items
+    if (!capture) {
+        pl.order = (int32_t *)ws.get(S_ORDER, sizeof(int32_t) * ((size_t)pl.item_cap + 1));
+        pl.order_hist = (unsigned long long *)ws.get(S_ORDER_HIST, sizeof(unsigned long long) * kOrderBuckets);
+    }
+    const uint32_t og = cap_grid(div_up((uint64_t)pl.item_cap, 256), (int64_t)ctx->cu_count * 8);
+    TR_LAUNCH(ctx, "item_order", k_fill_u64, div_up((uint64_t)kOrderBuckets, 256), 256, 0, pl.order_hist,
+              kOrderBuckets);
+    TR_LAUNCH(ctx, "item_order", k_item_hist, og, 256, 0, pl.items, pl.wins, pl.cnt, pl.order_hist);
+    TR_LAUNCH(ctx, "item_order", k_item_hist_scan, 1, 1024, 0, pl.order_hist);
+    TR_LAUNCH(ctx, "item_order", k_item_scatter, og, 256, 0, pl.items, pl.wins, pl.cnt, pl.order_hist, pl.order);
     launch_exclusive_scan_i32_to_i64(ctx, pl.clen, pl.cbase, pl.item_cap, pl.scan_tmp);
     if (capture) return pl.cursor_cap;
     return read_scalar<int64_t>(ctx, pl.cbase + pl.item_cap);
@@ -1156,10 +1253,10 @@ void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *dp, int np) {
         const uint32_t grid = heavy_grid(cap_grid(pl.h.items, 1 << 24));
         if (pl.tune.heavy_table == 4096)
             TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<4096, 512>), grid, 512, 0, pl.d_tasks, pl.nt, pl.items,
-                      pl.cnt, pl.wins, pl.cbase, pl.cursors, pl.rowptr, dp, np);
+                      pl.cnt, pl.wins, pl.order, pl.cbase, pl.cursors, pl.rowptr, dp, np);
         else
             TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<2048, 256>), grid, 256, 0, pl.d_tasks, pl.nt, pl.items,
-                      pl.cnt, pl.wins, pl.cbase, pl.cursors, pl.rowptr, dp, np);
+                      pl.cnt, pl.wins, pl.order, pl.cbase, pl.cursors, pl.rowptr, dp, np);
     }
 }
 

@@ -58,16 +58,22 @@ struct Request {
 
 // A finished batch; stays alive while a result or partial refers to it.  Its
 // admitted memory bound is returned to the budget when it is destroyed.
+// The worker's pool keeps freed blocks for reuse; beyond cache_cap bytes it
+// returns them to the device, so that the caches of all workers together stay
+// within what the budget leaves free.
 struct Batch {
     tr_batch_t b = nullptr;
     tr_ctx_t ctx = nullptr;
     std::mutex *ctx_mu = nullptr;
     std::atomic<size_t> *in_use = nullptr;
     size_t bound = 0;
+    size_t cache_cap = 0;
     ~Batch() {
         if (b) {
             std::lock_guard<std::mutex> lk(*ctx_mu);
             tr_batch_destroy(b);
+            size_t used = 0, cached = 0;
+            if (tr_pool_stats(ctx, &used, &cached) == TR_OK && cached > cache_cap) (void)tr_pool_trim(ctx);
         }
         if (in_use) in_use->fetch_sub(bound);
     }
@@ -206,16 +212,25 @@ void worker_loop(tr_server *s, Worker *w) {
         bt->ctx_mu = &w->mu;
         bt->in_use = &s->in_use;
         bt->bound = admitted;
+        bt->cache_cap = s->budget / (4 * s->workers.size());
         int rc;
         std::string err;
         int64_t fl = 0;
-        {
-            std::lock_guard<std::mutex> ck(w->mu);
-            rc = tr_batch_create(w->ctx, (int32_t)take.size(), As.data(), Bs.data(), nullptr, TR_BATCH_NO_GRAPH,
-                                 &bt->b);
-            if (rc == TR_OK) rc = tr_ctx_synchronize(w->ctx);
-            if (rc == TR_OK) tr_batch_stats(bt->b, &fl, nullptr, nullptr);
-            if (rc != TR_OK) err = tr_last_error(w->ctx);
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            {
+                std::lock_guard<std::mutex> ck(w->mu);
+                rc = tr_batch_create(w->ctx, (int32_t)take.size(), As.data(), Bs.data(), nullptr, TR_BATCH_NO_GRAPH,
+                                     &bt->b);
+                if (rc == TR_OK) rc = tr_ctx_synchronize(w->ctx);
+                if (rc == TR_OK) tr_batch_stats(bt->b, &fl, nullptr, nullptr);
+                if (rc != TR_OK) err = tr_last_error(w->ctx);
+            }
+            if (rc != TR_ERR_OOM || attempt) break;
+            // device memory held in the workers' pool caches: return it and retry once
+            for (auto &ow : s->workers) {
+                std::lock_guard<std::mutex> ck(ow->mu);
+                (void)tr_pool_trim(ow->ctx);
+            }
         }
         std::lock_guard<std::mutex> lk(s->mu);
         ++s->batches;

@@ -105,6 +105,8 @@ struct MulPlan {
     int32_t *clen = nullptr;
     int64_t *cbase = nullptr;
     int2 *cursors = nullptr;  // (cursor, column at the cursor) per A entry of multi-window items
+    int32_t *order = nullptr;          // heavy items sorted by the column of their first window
+    unsigned long long *order_hist = nullptr;
     int64_t cursor_cap = 0;
     int64_t *scan_tmp = nullptr;  // owned scan temp (graph mode)
     size_t scan_tmp_bytes = 0;

@@ -121,6 +121,8 @@ enum ScratchSlot : int {
     S_MISC2,
     S_PANELS,
     S_TMP_CSX,
+    S_ORDER,       // heavy items in column order
+    S_ORDER_HIST,  // column-bucket histogram / cursors of the item order
     S_NUM_SLOTS
 };
 
