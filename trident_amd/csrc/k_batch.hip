@@ -339,6 +339,16 @@ extern "C" {
 
 int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_csx_t *B, const tr_csx_t *C_in,
                     int32_t flags, tr_batch_t *out) {
+    return tr::batch_create(ctx, ntasks, A, B, C_in, flags, true, out);
+}
+
+}  // extern "C"
+
+// validate == false: the caller vouches for the operands (the server's
+// registered and produced operands), which skips one device check and host
+// read per task
+int tr::batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_csx_t *B, const tr_csx_t *C_in,
+                     int32_t flags, bool validate, tr_batch_t *out) {
     if (!ctx || !out) return TR_ERR_INVALID;
     *out = nullptr;
     tr_batch *b = nullptr;
@@ -356,7 +366,8 @@ int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_cs
         int32_t max_m = 0;
         for (int t = 0; t < ntasks; ++t) {
             b->Cin[t] = C_in ? C_in[t] : tr_csx_t{};
-            validate_operands(ctx, &b->A[t], &b->B[t]);
+            if (validate) validate_operands(ctx, &b->A[t], &b->B[t]);
+            else check_operand_shapes(&b->A[t], &b->B[t]);
             const tr_csx_t &C = b->Cin[t];
             if (C.ptr_vec) {
                 check_csx(&C, "C_in");
@@ -470,6 +481,8 @@ int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_cs
     *out = b;
     return TR_OK;
 }
+
+extern "C" {
 
 int tr_batch_run(tr_batch_t b) {
     if (!b) return TR_ERR_INVALID;

@@ -1283,11 +1283,15 @@ TaskDev task_of_pair(const tr_csx_t *A, const tr_csx_t *B) {
 
 }  // namespace
 
-void validate_operands(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B) {
+void check_operand_shapes(const tr_csx_t *A, const tr_csx_t *B) {
     check_csx(A, "A");
     check_csx(B, "B");
     TR_REQUIRE(A->majordim == TR_ROWS && B->majordim == TR_ROWS, "spgemm operands must be CSR (TR_ROWS)");
     TR_REQUIRE(A->ncols == B->nrows, "spgemm: A.ncols != B.nrows");
+}
+
+void validate_operands(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B) {
+    check_operand_shapes(A, B);
     int32_t *st = (int32_t *)scratch(ctx, S_TMP_CSX, 64);
     TR_HIP(hipMemsetAsync(st, 0, 2 * sizeof(int32_t), ctx->stream));
     launch_validate(ctx, A, st);

@@ -20,6 +20,8 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <array>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -103,6 +105,9 @@ struct tr_server {
     int64_t next_id = 1;
     std::atomic<size_t> in_use{0};
     bool stopping = false;
+    // symbolic memory bound per (kind, operand a, operand b, step): operands
+    // are resident and products deterministic, so a bound is computed once
+    std::map<std::array<int64_t, 4>, size_t> bounds;
     int64_t completed = 0, batches = 0, flops_done = 0, failed = 0;
     std::vector<double> lat_ms;
     Clock::time_point t0 = Clock::now();
@@ -169,29 +174,46 @@ void worker_loop(tr_server *s, Worker *w) {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv.wait(lk, [&] { return s->stopping || !s->queue.empty(); });
             if (s->stopping && s->queue.empty()) return;
-            // admission in arrival order while the symbolic bound fits
+            // admission in arrival order while the symbolic bound fits; a step
+            // whose bound is not known yet is taken out, its bound computed
+            // outside the server lock, and put back at the front
             while (!s->queue.empty() && (int32_t)take.size() < s->max_batch) {
                 Request *r = s->queue.front();
-                tr_csx_t A, B;
-                step_operands(s, r, A, B);
-                int64_t nnz = 0;
-                size_t bytes = 0;
-                int rc;
-                {
-                    std::lock_guard<std::mutex> ck(w->mu);
-                    rc = tr_symbolic(w->ctx, &A, &B, &nnz, &bytes);
-                }
-                if (rc != TR_OK) {
+                const std::array<int64_t, 4> key{r->kind, r->a, r->b, r->step};
+                auto it = s->bounds.find(key);
+                if (it == s->bounds.end()) {
+                    if (!take.empty()) break;  // run what is admitted; bound it next time
+                    tr_csx_t A, B;
+                    step_operands(s, r, A, B);
                     s->queue.pop_front();
-                    finish_step(s, r, nullptr, 0, rc, tr_last_error(w->ctx));
+                    lk.unlock();
+                    int64_t nnz = 0;
+                    size_t bytes = 0;
+                    int rc;
+                    std::string err;
+                    {
+                        std::lock_guard<std::mutex> ck(w->mu);
+                        rc = tr_symbolic(w->ctx, &A, &B, &nnz, &bytes);
+                        if (rc != TR_OK) err = tr_last_error(w->ctx);
+                    }
+                    lk.lock();
+                    if (rc != TR_OK) {
+                        finish_step(s, r, nullptr, 0, rc, err);
+                        continue;
+                    }
+                    s->bounds[key] = bytes;
+                    s->queue.push_front(r);
                     continue;
                 }
+                const size_t bytes = it->second;
                 if (bytes > s->budget) {
                     s->queue.pop_front();
                     finish_step(s, r, nullptr, 0, TR_ERR_OOM, "request exceeds the serving memory budget");
                     continue;
                 }
                 if (s->in_use.load() + admitted + bytes > s->budget) break;  // wait for memory to free up
+                tr_csx_t A, B;
+                step_operands(s, r, A, B);
                 s->queue.pop_front();
                 r->bound = bytes;
                 admitted += bytes;
@@ -219,8 +241,8 @@ void worker_loop(tr_server *s, Worker *w) {
         for (int attempt = 0; attempt < 2; ++attempt) {
             {
                 std::lock_guard<std::mutex> ck(w->mu);
-                rc = tr_batch_create(w->ctx, (int32_t)take.size(), As.data(), Bs.data(), nullptr, TR_BATCH_NO_GRAPH,
-                                     &bt->b);
+                rc = tr::batch_create(w->ctx, (int32_t)take.size(), As.data(), Bs.data(), nullptr, TR_BATCH_NO_GRAPH,
+                                      false, &bt->b);
                 if (rc == TR_OK) rc = tr_ctx_synchronize(w->ctx);
                 if (rc == TR_OK) tr_batch_stats(bt->b, &fl, nullptr, nullptr);
                 if (rc != TR_OK) err = tr_last_error(w->ctx);
@@ -274,6 +296,14 @@ int tr_server_create(int device, size_t mem_budget, int32_t workers, int32_t max
 
 int tr_server_register(tr_server_t s, const tr_csx_t *M, int32_t *op_id) {
     if (!s || !M || !op_id || M->majordim != TR_ROWS || !M->ptr_vec) return TR_ERR_INVALID;
+    {
+        // validated once here: the server's batches do not check operands again
+        std::lock_guard<std::mutex> ak(s->admin_mu);
+        int32_t st = 0;
+        int rc = tr_csx_validate(s->admin, M, &st);
+        if (rc != TR_OK) return rc;
+        if (st != 0) return TR_ERR_INVALID;
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     Operand o;
     o.m = *M;
