@@ -164,6 +164,7 @@ struct tr_batch {
     std::vector<int64_t> o_nnz;
     int64_t flops = 0;
     bool no_accum = false;  // no task has an accumulator: results are the products
+    bool single_run = false;  // workspace in the context's scratch slots: no replay
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool ran = false;
@@ -235,7 +236,7 @@ void batch_body(tr_batch *b, bool capture) {
         mul_read_counts(ctx, b->mp);
         if (b->mp.h.overflow & 1) fail(TR_ERR_HIP, "batch: heavy work list capacity exceeded");
         b->mp.cursor_cap = ncur;
-        if (b->mp.h.cls[kHeavyClass] && b->mp.h.items > 0)
+        if (b->mp.h.items > 0)
             b->mp.cursors = (int2 *)b->ws_mul->get(S_CURSORS, sizeof(int2) * (size_t)std::max<int64_t>(ncur, 1));
         b->p_idx = (int32_t *)b->own(sizeof(int32_t) * (size_t)std::max<int64_t>(b->p_total, 1));
         b->p_val = (double *)b->own(sizeof(double) * (size_t)std::max<int64_t>(b->p_total, 1));
@@ -354,7 +355,9 @@ int tr::batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_c
     tr_batch *b = nullptr;
     const int rc = bguard(ctx, [&] {
         TR_REQUIRE(ntasks >= 1 && A && B, "tr_batch_create: need at least one task");
-        TR_REQUIRE((flags & ~(TR_BATCH_COMPRESS | TR_BATCH_NO_GRAPH)) == 0, "tr_batch_create: unknown flags");
+        TR_REQUIRE((flags & ~(TR_BATCH_COMPRESS | TR_BATCH_NO_GRAPH | kBatchSingleRun)) == 0,
+                   "tr_batch_create: unknown flags");
+        TR_REQUIRE(!(flags & kBatchSingleRun) || (flags & TR_BATCH_NO_GRAPH), "batch: single-run batches have no graph");
         const int32_t compress_mode = (flags & TR_BATCH_COMPRESS) ? 1 : 0;
         b = new tr_batch();
         b->ctx = ctx;
@@ -430,7 +433,11 @@ int tr::batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_c
                 T.bv = bb.val;
             }
         }
-        b->ws_mul.reset(new Workspace(ctx, true));
+        // single-run batches without accumulators (the server's) take their
+        // multiply workspace from the context's scratch slots instead of
+        // allocating one per batch
+        b->single_run = (flags & kBatchSingleRun) && b->no_accum;
+        b->ws_mul.reset(new Workspace(ctx, !b->single_run));
         b->ws_add.reset(new Workspace(ctx, true));
         mul_setup(ctx, *b->ws_mul, b->mp, tasks);
         // eager run: sizes every list and output
@@ -488,6 +495,7 @@ int tr_batch_run(tr_batch_t b) {
     if (!b) return TR_ERR_INVALID;
     tr_ctx *ctx = b->ctx;
     return bguard(ctx, [&] {
+        TR_REQUIRE(!b->single_run, "tr_batch_run: this batch was created for a single run");
         if (b->exec) {
             ::tr::TimerScope ts(ctx, "batch_graph");
             TR_HIP(hipGraphLaunch(b->exec, ctx->stream));

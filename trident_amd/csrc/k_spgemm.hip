@@ -238,12 +238,12 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_row_flops(const TaskDev *__re
                 ++c;
             }
         } else {
-            c = kHeavyClass;
+            c = f <= kMidMaxF ? kMidClass : kHeavyClass;
         }
         lpos = atomicAdd(&s_cnt[c], 1);
     }
     const unsigned long long wf = (unsigned long long)wave_sum(f);
-    const unsigned long long whf = (unsigned long long)wave_sum(c == kHeavyClass ? f : (int64_t)0);
+    const unsigned long long whf = (unsigned long long)wave_sum(c >= kHeavyClass ? f : (int64_t)0);
     if (l == 0) {
         atomicAdd(&s_f, wf);
         atomicAdd(&s_hf, whf);
@@ -317,14 +317,16 @@ __global__ __launch_bounds__(256) void k_sym_light(const TaskDev *__restrict__ t
 // of 2^20 columns in LDS per pass, popcounts per 128-column fine block, and
 // the row's column windows (<= win_h + 127 entries, <= kWinMaxWidth columns)
 // appended to the device work lists.
-__global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
-    const TaskDev *__restrict__ tasks, int nt, const int32_t *__restrict__ rows, Counters *__restrict__ cnt,
+template <int PB, int NTH>
+__global__ __launch_bounds__(NTH) void k_sym_heavy(
+    const TaskDev *__restrict__ tasks, int nt, const int32_t *__restrict__ rows, Counters *__restrict__ cnt, int cls,
     int64_t *__restrict__ rownnz, int emit, int32_t win_h, int32_t item_windows, Window *__restrict__ wins,
     int32_t win_cap, Item *__restrict__ items, int32_t item_cap) {
-    constexpr int kWords = 1 << (kPassBits - 5);          // 32768 words
-    constexpr int kFine = 1 << (kPassBits - kFineShift);  // 8192 fine blocks
-    constexpr int kPerThread = kFine / kHeavyThreads;     // 8
-    constexpr int kWavesB = kHeavyThreads / kWave;        // 16
+    constexpr int kWords = 1 << (PB - 5);          // bitmap words of one pass
+    constexpr int kFine = 1 << (PB - kFineShift);  // fine blocks of one pass
+    constexpr int kPerThread = kFine / NTH;        // 8
+    constexpr int kWavesB = NTH / kWave;
+    static_assert(kPerThread == 8, "eight fine blocks per thread");
     __shared__ uint32_t bm[kWords];
     __shared__ int32_t wsum[kWavesB];
     __shared__ int32_t sh_wbase;
@@ -332,7 +334,7 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
     const int tid = threadIdx.x, w = tid / kWave, l = lane_id();
     int32_t *fex_lds = (int32_t *)bm;            // [0, kFine) after marking
     int32_t *first_lds = (int32_t *)bm + kFine;  // [kFine, 2 kFine)
-    const int32_t nheavy = cnt->cls[kHeavyClass];
+    const int32_t nheavy = cnt->cls[cls];
     for (int32_t hi = blockIdx.x; hi < nheavy; hi += gridDim.x) {
         const int32_t g = rows[hi];
         const TaskDev &TK = tasks[task_of(tasks, nt, g)];
@@ -343,11 +345,11 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
         const int32_t ncols = TK.n;
         const int32_t a0 = TK.ap[r], a1 = TK.ap[r + 1];
         int32_t row_total = 0;
-        for (int64_t base = 0; base < ncols; base += (1 << kPassBits)) {
-            const int32_t width = (int32_t)min<int64_t>(1 << kPassBits, (int64_t)ncols - base);
+        for (int64_t base = 0; base < ncols; base += (1 << PB)) {
+            const int32_t width = (int32_t)min<int64_t>(1 << PB, (int64_t)ncols - base);
             const int32_t nwords = (width + 31) >> 5;
             const int32_t nfine = (width + (1 << kFineShift) - 1) >> kFineShift;
-            for (int t = tid; t < kWords; t += kHeavyThreads) bm[t] = 0u;
+            for (int t = tid; t < kWords; t += NTH) bm[t] = 0u;
             __syncthreads();
             // mark: entry pb + w + 16 t goes to wave w
             for (int32_t pb = a0; pb < a1; pb += kWavesB * kWave) {
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(kHeavyThreads) void k_sym_heavy(
             __syncthreads();
             const int32_t wbase = sh_wbase;
             if (wbase >= 0) {
-                for (int32_t rr = tid; rr < nwin; rr += kHeavyThreads) {
+                for (int32_t rr = tid; rr < nwin; rr += NTH) {
                     const int32_t f = first_lds[rr];
                     const int32_t fn = (rr + 1 < nwin) ? first_lds[rr + 1] : nfine;
                     Window W;
@@ -1188,13 +1190,13 @@ void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool captu
     if (pl.h.cls[3]) launch_sym_light<512>(ctx, pl, 3);
     if (pl.h.cls[4]) launch_sym_light<1024>(ctx, pl, 4);
     if (pl.h.cls[5]) launch_sym_light<2048>(ctx, pl, 5);
-    const int32_t nh = pl.h.cls[kHeavyClass];
+    const int32_t nh = pl.h.cls[kHeavyClass] + pl.h.cls[kMidClass];
     if (nh) {
         if (emit && !capture) {
             int32_t max_n = 0;
             for (auto &t : pl.h_tasks) max_n = std::max(max_n, t.n);
-            // windows per pass <= ceil(pass_nnz / H) + width splits
-            const int64_t passes = ((int64_t)max_n + (1 << kPassBits) - 1) >> kPassBits;
+            // windows per pass <= ceil(pass_nnz / H) + width splits (mid rows use the shorter pass)
+            const int64_t passes = ((int64_t)max_n + (1 << kMidPassBits) - 1) >> kMidPassBits;
             const int64_t wsplits = (int64_t)max_n / kWinMaxWidth + 1;
             const int64_t q = std::min<int64_t>(pl.tune.window_entries, kSparseH);
             const int64_t wcap = (int64_t)(pl.h.heavy_flops / q) + (int64_t)nh * (2 * passes + wsplits) + 16;
@@ -1205,16 +1207,23 @@ void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool captu
             pl.wins = (Window *)ws.get(S_WINDOWS, sizeof(Window) * (size_t)wcap);
             pl.items = (Item *)ws.get(S_ITEMS, sizeof(Item) * (size_t)icap);
         }
-        TR_LAUNCH(ctx, "sym_heavy", k_sym_heavy, cap_grid(nh, (int64_t)ctx->cu_count * 2), kHeavyThreads, 0,
-                  pl.d_tasks, pl.nt, pl.lists + (int64_t)kHeavyClass * pl.rows_total, pl.cnt, pl.rownnz,
-                  emit ? 1 : 0, pl.tune.window_entries, pl.tune.item_windows, pl.wins, pl.win_cap, pl.items,
-                  pl.item_cap);
+        if (pl.h.cls[kHeavyClass])
+            TR_LAUNCH(ctx, "sym_heavy", (k_sym_heavy<kPassBits, kHeavyThreads>),
+                      cap_grid(pl.h.cls[kHeavyClass], (int64_t)ctx->cu_count * 2), kHeavyThreads, 0, pl.d_tasks,
+                      pl.nt, pl.lists + (int64_t)kHeavyClass * pl.rows_total, pl.cnt, kHeavyClass, pl.rownnz,
+                      emit ? 1 : 0, pl.tune.window_entries, pl.tune.item_windows, pl.wins, pl.win_cap, pl.items,
+                      pl.item_cap);
+        if (pl.h.cls[kMidClass])
+            TR_LAUNCH(ctx, "sym_heavy", (k_sym_heavy<kMidPassBits, kMidThreads>),
+                      cap_grid(pl.h.cls[kMidClass], (int64_t)ctx->cu_count * 4), kMidThreads, 0, pl.d_tasks, pl.nt,
+                      pl.lists + (int64_t)kMidClass * pl.rows_total, pl.cnt, kMidClass, pl.rownnz, emit ? 1 : 0,
+                      pl.tune.window_entries, pl.tune.item_windows, pl.wins, pl.win_cap, pl.items, pl.item_cap);
     }
     launch_exclusive_scan_i64(ctx, pl.rownnz, pl.rowptr, pl.rows_total, pl.scan_tmp);
 }
 
 int64_t mul_cursor_prep(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool capture) {
-    if (!pl.h.cls[kHeavyClass]) return 0;
+    if (!pl.h.cls[kHeavyClass] && !pl.h.cls[kMidClass]) return 0;
     if (!capture) {
         pl.clen = (int32_t *)ws.get(S_MISC, sizeof(int32_t) * ((size_t)pl.item_cap + 1));
         pl.cbase = (int64_t *)ws.get(S_MISC2, sizeof(int64_t) * ((size_t)pl.item_cap + 1));
@@ -1249,7 +1258,7 @@ void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *dp, int np) {
     if (pl.h.cls[3]) launch_num_light<512>(ctx, pl, 3, dp, np);
     if (pl.h.cls[4]) launch_num_light<1024>(ctx, pl, 4, dp, np);
     if (pl.h.cls[5]) launch_num_light<2048>(ctx, pl, 5, dp, np);
-    if (pl.h.cls[kHeavyClass] && pl.h.items > 0) {
+    if (pl.h.items > 0) {
         const uint32_t grid = heavy_grid(cap_grid(pl.h.items, 1 << 24));
         if (pl.tune.heavy_table == 4096)
             TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<4096, 512>), grid, 512, 0, pl.d_tasks, pl.nt, pl.items,
@@ -1392,7 +1401,7 @@ int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max
     *npanels = np;
     Panel *dp = (Panel *)ws.get(S_TMP_CSX, sizeof(Panel) * (size_t)np);
     TR_HIP(hipMemcpyAsync(dp, hp.data(), sizeof(Panel) * (size_t)np, hipMemcpyHostToDevice, ctx->stream));
-    if (pl.h.cls[kHeavyClass] && pl.h.items > 0)
+    if (pl.h.items > 0)
         pl.cursors = (int2 *)ws.get(S_CURSORS, sizeof(int2) * (size_t)std::max<int64_t>(ncur, 1));
     mul_numeric(ctx, pl, dp, np);
     mul_check(ctx, pl);

@@ -60,9 +60,10 @@ struct Request {
 
 // A finished batch; stays alive while a result or partial refers to it.  Its
 // admitted memory bound is returned to the budget when it is destroyed.
-// The worker's pool keeps freed blocks for reuse; beyond cache_cap bytes it
-// returns them to the device, so that the caches of all workers together stay
-// within what the budget leaves free.
+// The worker's pool keeps freed blocks for reuse (allocation and release of
+// large blocks are expensive and releases synchronise the device); beyond its
+// share of the budget it returns them.  An allocation that still fails trims
+// every worker's pool and retries (worker_loop).
 struct Batch {
     tr_batch_t b = nullptr;
     tr_ctx_t ctx = nullptr;
@@ -234,15 +235,15 @@ void worker_loop(tr_server *s, Worker *w) {
         bt->ctx_mu = &w->mu;
         bt->in_use = &s->in_use;
         bt->bound = admitted;
-        bt->cache_cap = s->budget / (4 * s->workers.size());
+        bt->cache_cap = s->budget / s->workers.size();
         int rc;
         std::string err;
         int64_t fl = 0;
         for (int attempt = 0; attempt < 2; ++attempt) {
             {
                 std::lock_guard<std::mutex> ck(w->mu);
-                rc = tr::batch_create(w->ctx, (int32_t)take.size(), As.data(), Bs.data(), nullptr, TR_BATCH_NO_GRAPH,
-                                      false, &bt->b);
+                rc = tr::batch_create(w->ctx, (int32_t)take.size(), As.data(), Bs.data(), nullptr,
+                                      TR_BATCH_NO_GRAPH | tr::kBatchSingleRun, false, &bt->b);
                 if (rc == TR_OK) rc = tr_ctx_synchronize(w->ctx);
                 if (rc == TR_OK) tr_batch_stats(bt->b, &fl, nullptr, nullptr);
                 if (rc != TR_OK) err = tr_last_error(w->ctx);

@@ -16,8 +16,12 @@ namespace tr {
 // tuning constants (see DESIGN notes in the repository documentation)
 constexpr int kLightMaxFCap = 1024; // largest light/heavy cut (light tables reach 2048 slots)
 constexpr int kNumLight = 6;       // light classes: T = 64 << c, c < kNumLight
-constexpr int kHeavyClass = kNumLight;
-constexpr int kNumClasses = kNumLight + 1;
+constexpr int kHeavyClass = kNumLight;      // heavy rows with more than kMidMaxF multiplications
+constexpr int kMidClass = kNumLight + 1;    // heavy rows up to kMidMaxF multiplications
+constexpr int kNumClasses = kNumLight + 2;
+constexpr int kMidMaxF = 16384;
+constexpr int kMidPassBits = 19;            // mid rows: 2^19-column passes, two blocks per CU
+constexpr int kMidThreads = 512;
 constexpr int kPassBits = 20;      // heavy bitmap covers 2^20 columns per pass
 constexpr int kFineShift = 7;      // fine block = 128 columns
 constexpr int kWinMaxWidth = 32768;                       // window width cap (columns)
@@ -64,7 +68,6 @@ struct TaskDev {
 // device counters written by the symbolic phase
 struct Counters {
     int32_t cls[kNumClasses];        // rows per class
-    int32_t pad0;
     unsigned long long flops;        // total multiplications
     unsigned long long heavy_flops;  // multiplications of heavy rows
     int32_t windows;                 // windows appended

@@ -186,6 +186,9 @@ void launch_compress_fixed(tr_ctx *ctx, int32_t n, const int32_t *ptr, const int
 // k_spgemm.hip
 void validate_operands(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B);
 void check_operand_shapes(const tr_csx_t *A, const tr_csx_t *B);  // host-side shape checks only
+// internal batch flag (not part of the C-ABI): a single run, workspace in
+// the context's scratch slots (valid only without accumulators)
+constexpr int32_t kBatchSingleRun = 1 << 30;
 int batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_csx_t *B, const tr_csx_t *C_in,
                  int32_t flags, bool validate, tr_batch_t *out);
 int64_t spgemm_flops(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B);
