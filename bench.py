@@ -479,7 +479,8 @@ def main():
                     help="multiplications in the CPU-baseline row sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
-    ap.add_argument("--serve-workers", type=int, default=3)
+    ap.add_argument("--serve-workers", type=int, default=1,
+                    help="server worker threads (one batch at a time keeps the GPU's caches for one product)")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.5, 1.0, 4.0])
     ap.add_argument("--tuning", default="auto",
                     help="auto (trident_amd/tuning/<workload>.json if present), none, or a JSON path")

@@ -195,7 +195,10 @@ int tr_batch_destroy(tr_batch_t batch);
  * `workers` threads, each with its own stream, admit waiting steps in
  * arrival order while their symbolic memory bound (tr_symbolic) fits
  * `mem_budget`, and run them as one batch of up to `max_batch` steps;
- * batches of different workers overlap on the device.  Results are
+ * steps that arrive while a batch runs join the next one.  One worker is
+ * the efficient choice on one GPU: batches of several workers overlap on
+ * the device, but concurrent products compete for its caches (measured
+ * 3.4x lower throughput with two workers on MI355X).  Results are
  * bit-identical to the same product run alone.  No counterpart in the
  * reference, which has no serving layer. */
 #define TR_REQ_SPGEMM 0

@@ -52,7 +52,7 @@ TUNING_DIR = os.path.join(HERE, "tuning")
 
 # search space
 TABLE_WINDOWS = [(2048, 384), (2048, 640), (2048, 896), (4096, 1408), (4096, 1920)]
-ITEM_WINDOWS = [4, 8, 16, 32]
+ITEM_WINDOWS = [1, 2, 4, 8, 16, 32]
 LIGHT_MAX = [256, 512, 1024]
 
 

@@ -48,6 +48,7 @@ void *Pool::alloc(size_t bytes) {
             fail(TR_ERR_OOM, "hipMalloc of " + std::to_string(b) + " bytes failed");
         }
     }
+    ++n_malloc;
     live_[p] = b;
     in_use_ += b;
     return p;
@@ -70,6 +71,7 @@ size_t Pool::size_of(void *p) const {
 }
 
 void Pool::trim() {
+    n_free += free_.size();
     for (auto &kv : free_) (void)hipFree(kv.second);
     free_.clear();
     cached_ = 0;

@@ -239,6 +239,10 @@ void worker_loop(tr_server *s, Worker *w) {
         int rc;
         std::string err;
         int64_t fl = 0;
+#ifdef TR_SERVER_TRACE
+        const auto tb0 = Clock::now();
+        const size_t nm0 = w->ctx->pool.n_malloc, nf0 = w->ctx->pool.n_free;
+#endif
         for (int attempt = 0; attempt < 2; ++attempt) {
             {
                 std::lock_guard<std::mutex> ck(w->mu);
@@ -255,6 +259,12 @@ void worker_loop(tr_server *s, Worker *w) {
                 (void)tr_pool_trim(ow->ctx);
             }
         }
+#ifdef TR_SERVER_TRACE
+        fprintf(stderr, "[srv] worker %p tasks %zu bound %.1f MB flops %.3e batch %.2f ms mallocs %zu frees %zu rc %d\n",
+                (void *)w, take.size(), admitted / 1e6, (double)fl,
+                std::chrono::duration<double, std::milli>(Clock::now() - tb0).count(),
+                w->ctx->pool.n_malloc - nm0, w->ctx->pool.n_free - nf0, rc);
+#endif
         std::lock_guard<std::mutex> lk(s->mu);
         ++s->batches;
         s->flops_done += fl;

@@ -51,6 +51,7 @@ class Pool {
     void trim();
     size_t in_use() const { return in_use_; }
     size_t cached() const { return cached_; }
+    size_t n_malloc = 0, n_free = 0;  // device allocations / releases made (not cache hits)
     void set_limit(size_t bytes) { limit_ = bytes; }
 
   private:
