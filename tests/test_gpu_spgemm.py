@@ -5,6 +5,7 @@ The bar: row pointers and column indices bit-exact (sorted columns) and
 values bit-exact — the HIP kernels apply every product in the oracle's
 canonical order, so max relative error is 0 (the contract allows 1e-12).
 """
+import ctypes
 import os
 
 import numpy as np
@@ -158,6 +159,51 @@ def test_heavy_sparse_rows(ctx, ncols):
     aptr = np.concatenate([[0], np.cumsum([len(x) for x in al])])
     A = gen.from_arrays(96, k, aptr, np.concatenate(al), rng.uniform(-1, 1, aptr[-1]))
     run(ctx, A, B, f"sparse heavy {ncols}")
+
+
+def _device_bytes_equal(ctx, pa, pb, nbytes, chunk=1 << 29):
+    """Byte equality of two device ranges, copied to the host in chunks."""
+    ha = np.empty(max(1, min(chunk, nbytes)), np.uint8)
+    hb = np.empty_like(ha)
+    off = 0
+    while off < nbytes:
+        n = min(chunk, nbytes - off)
+        ctx.check(ctx.lib.tr_memcpy_d2h(ctx.h, ha.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(pa + off), n))
+        ctx.check(ctx.lib.tr_memcpy_d2h(ctx.h, hb.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(pb + off), n))
+        if not np.array_equal(ha[:n], hb[:n]):
+            return False
+        off += n
+    return True
+
+
+def test_heavy_items_beyond_grid_cap(ctx):
+    # R-MAT scale 19 with 64-entry windows, one window per heavy item.  A
+    # window holds fewer than 384 entries, so (heavy-row entries) / 384
+    # bounds the item count from below; it exceeds 2^22, the cap of the heavy
+    # numeric grid (one workgroup per item, and a dispatch counts its
+    # work-items in 32 bits), so workgroups stride over the items.  Every
+    # cutover keeps the result bit-identical: it must equal the default
+    # tuning's panels.
+    A = gen.rmat(19, 16, seed=1)
+    dA = ctx.upload(A)
+    nnz, _ = symbolic(dA, dA)
+    light_entries_max = A.nrows * 1024  # a light row has <= 1024 products
+    assert (nnz - light_entries_max) // 384 > 2 ** 22
+    ref, rb_ref = spgemm_panels(dA, dA, max_panels=64)
+    ctx.set_tuning(dict(ctx.get_tuning(), window_entries=64, item_windows=1))
+    try:
+        got, rb_got = spgemm_panels(dA, dA, max_panels=64)
+    finally:
+        ctx.set_tuning(None)
+    assert rb_got == rb_ref and sum(p.nnz for p in got) == nnz
+    for p, q in zip(got, ref):
+        assert (p.d.nrows, p.nnz) == (q.d.nrows, q.nnz)
+        assert _device_bytes_equal(ctx, p.d.ptr_vec, q.d.ptr_vec, 4 * (p.d.nrows + 1)), "row pointers differ"
+        assert _device_bytes_equal(ctx, p.d.idx_vec, q.d.idx_vec, 4 * p.nnz), "column indices differ"
+        assert _device_bytes_equal(ctx, p.d.val, q.d.val, 8 * p.nnz), "values differ"
+    for p in got + ref:
+        p.free()
+    dA.free()
 
 
 def test_structural_zero_and_unsorted_A(ctx):

@@ -1259,7 +1259,9 @@ void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *dp, int np) {
     if (pl.h.cls[4]) launch_num_light<1024>(ctx, pl, 4, dp, np);
     if (pl.h.cls[5]) launch_num_light<2048>(ctx, pl, 5, dp, np);
     if (pl.h.items > 0) {
-        const uint32_t grid = heavy_grid(cap_grid(pl.h.items, 1 << 24));
+        // one workgroup per item, grid-stride beyond 2^22 workgroups (the
+        // dispatch packet counts work-items in 32 bits)
+        const uint32_t grid = heavy_grid(cap_grid(pl.h.items, 1 << 22));
         if (pl.tune.heavy_table == 4096)
             TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<4096, 512>), grid, 512, 0, pl.d_tasks, pl.nt, pl.items,
                       pl.cnt, pl.wins, pl.order, pl.cbase, pl.cursors, pl.rowptr, dp, np);

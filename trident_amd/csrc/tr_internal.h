@@ -143,6 +143,8 @@ void fold_timing(tr_ctx *ctx);
 #define TR_LAUNCH(ctx, name, kernel, grid, block, shmem, ...)                                     \
     do {                                                                                          \
         dim3 _g(grid), _b(block);                                                                 \
+        if ((uint64_t)_g.x * _b.x > 0xffffffffull)                                                \
+            ::tr::fail(TR_ERR_HIP, std::string("launch of ") + (name) + ": grid exceeds 2^32 work-items"); \
         if (_g.x > 0 && _g.y > 0 && _g.z > 0) {                                                   \
             ::tr::TimerScope _ts((ctx), (name));                                                  \
             hipLaunchKernelGGL(kernel, _g, _b, (shmem), (ctx)->stream, __VA_ARGS__);               \
