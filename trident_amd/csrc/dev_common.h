@@ -13,6 +13,16 @@ constexpr int kWave = 64;
 constexpr int32_t kEmpty = -1;          // empty hash slot (columns are >= 0)
 constexpr int32_t kKeyPad = 0x7fffffff;  // sort padding key
 
+// Global-memory (address space 1) view of a device pointer.  Accesses
+// through it are global instructions, which count on vmcnt only; a flat
+// access (what a pointer read from memory compiles to) also counts on
+// lgkmcnt, so every wait for an LDS result would wait for it as well.
+#define TR_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ TR_GLOBAL T *as_global(T *p) {
+    return (TR_GLOBAL T *)p;
+}
+
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
 // Order this wave's LDS accesses across the call (and wait for outstanding

@@ -44,7 +44,7 @@ namespace {
 
 using namespace dev;
 
-__device__ __forceinline__ int32_t lower_bound_dev(const int32_t *__restrict__ a, int32_t lo, int32_t hi,
+__device__ __forceinline__ int32_t lower_bound_dev(const TR_GLOBAL int32_t *__restrict__ a, int32_t lo, int32_t hi,
                                                    int32_t key) {
     while (lo < hi) {
         int32_t mid = lo + ((hi - lo) >> 1);
@@ -57,8 +57,8 @@ __device__ __forceinline__ int32_t lower_bound_dev(const int32_t *__restrict__ a
 // first q in (lo, e] with a[q] >= key (q == e if none), galloping from lo,
 // where a[lo] < key is known; vq receives a[q] (INT32_MAX when q == e), the
 // column the next window of a heavy row starts from
-__device__ __forceinline__ int32_t gallop_from(const int32_t *__restrict__ a, int32_t lo, int32_t e, int32_t key,
-                                               int32_t &vq) {
+__device__ __forceinline__ int32_t gallop_from(const TR_GLOBAL int32_t *__restrict__ a, int32_t lo, int32_t e,
+                                               int32_t key, int32_t &vq) {
     int32_t step = 1, hi;
     vq = INT32_MAX;
     while (true) {
@@ -92,7 +92,7 @@ __device__ __forceinline__ int32_t gallop_from(const int32_t *__restrict__ a, in
 // Visit the column indices of B row segment [s, e) with the wave's lanes,
 // four loads in flight per lane before the first is consumed.
 template <typename F>
-__device__ __forceinline__ void stream_cols(const int32_t *__restrict__ bi, int32_t s, int32_t e, F &&f) {
+__device__ __forceinline__ void stream_cols(const TR_GLOBAL int32_t *__restrict__ bi, int32_t s, int32_t e, F &&f) {
     int32_t q = s + lane_id();
     for (; q + 3 * kWave < e; q += 4 * kWave) {
         const int32_t j0 = bi[q], j1 = bi[q + kWave], j2 = bi[q + 2 * kWave], j3 = bi[q + 3 * kWave];
@@ -109,23 +109,25 @@ __device__ __forceinline__ void stream_cols(const int32_t *__restrict__ bi, int3
 // Alignment is taken from the address: caller tiles may place indices at any
 // 4-byte offset.
 template <typename F>
-__device__ __forceinline__ void stream_cols_v4(const int32_t *__restrict__ bi, int32_t s, int32_t e, F &&f) {
+__device__ __forceinline__ void stream_cols_v4(const TR_GLOBAL int32_t *__restrict__ bi, int32_t s, int32_t e,
+                                               F &&f) {
     const int l = lane_id();
     const int32_t mis = (int32_t)(((uintptr_t)(bi + s) >> 2) & 3);  // dwords past a 16-byte boundary
     const int32_t sa = min(e, s + ((4 - mis) & 3));                  // first 16-byte aligned index
     if (s + l < sa) f(bi[s + l]);
     const int32_t nv = (e - sa) >> 2;                                // whole int4 groups
-    const int4 *__restrict__ b4 = reinterpret_cast<const int4 *>(bi + sa);
+    typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+    const TR_GLOBAL i32x4 *__restrict__ b4 = (const TR_GLOBAL i32x4 *)(bi + sa);
     int32_t v = l;
     for (; v + 3 * kWave < nv; v += 4 * kWave) {
-        const int4 x0 = b4[v], x1 = b4[v + kWave], x2 = b4[v + 2 * kWave], x3 = b4[v + 3 * kWave];
+        const i32x4 x0 = b4[v], x1 = b4[v + kWave], x2 = b4[v + 2 * kWave], x3 = b4[v + 3 * kWave];
         f(x0.x), f(x0.y), f(x0.z), f(x0.w);
         f(x1.x), f(x1.y), f(x1.z), f(x1.w);
         f(x2.x), f(x2.y), f(x2.z), f(x2.w);
         f(x3.x), f(x3.y), f(x3.z), f(x3.w);
     }
     for (; v < nv; v += kWave) {
-        const int4 x = b4[v];
+        const i32x4 x = b4[v];
         f(x.x), f(x.y), f(x.z), f(x.w);
     }
     const int32_t ta = sa + (nv << 2);
@@ -135,8 +137,8 @@ __device__ __forceinline__ void stream_cols_v4(const int32_t *__restrict__ bi, i
 // Same with the values (all entries of one B row have distinct columns, so
 // the order in which one lane applies them does not matter).
 template <typename F>
-__device__ __forceinline__ void stream_entries(const int32_t *__restrict__ bi, const double *__restrict__ bv,
-                                               int32_t s, int32_t e, F &&f) {
+__device__ __forceinline__ void stream_entries(const TR_GLOBAL int32_t *__restrict__ bi,
+                                               const TR_GLOBAL double *__restrict__ bv, int32_t s, int32_t e, F &&f) {
     int32_t q = s + lane_id();
     for (; q + kWave < e; q += 2 * kWave) {
         const int32_t j0 = bi[q], j1 = bi[q + kWave];
@@ -275,12 +277,12 @@ __global__ __launch_bounds__(256) void k_sym_light(const TaskDev *__restrict__ t
         const int32_t g = rows[li];
         const TaskDev &TK = tasks[task_of(tasks, nt, g)];
         const int32_t r = g - TK.row_base;
-        const int32_t *__restrict__ ai = TK.ai;
-        const int32_t *__restrict__ bp = TK.bp;
-        const int32_t *__restrict__ bi = TK.bi;
+        const TR_GLOBAL int32_t *__restrict__ ai = as_global(TK.ai);
+        const TR_GLOBAL int32_t *__restrict__ bp = as_global(TK.bp);
+        const TR_GLOBAL int32_t *__restrict__ bi = as_global(TK.bi);
         for (int t = l; t < T; t += kWave) tab[t] = kEmpty;
         wave_sync();
-        const int32_t a0 = TK.ap[r], a1 = TK.ap[r + 1];
+        const int32_t a0 = as_global(TK.ap)[r], a1 = as_global(TK.ap)[r + 1];
         int cntv = 0;
         for (int32_t pb = a0; pb < a1; pb += kWave) {
             int32_t bs = 0, be = 0;
@@ -339,11 +341,11 @@ __global__ __launch_bounds__(NTH) void k_sym_heavy(
         const int32_t g = rows[hi];
         const TaskDev &TK = tasks[task_of(tasks, nt, g)];
         const int32_t r = g - TK.row_base;
-        const int32_t *__restrict__ ai = TK.ai;
-        const int32_t *__restrict__ bp = TK.bp;
-        const int32_t *__restrict__ bi = TK.bi;
+        const TR_GLOBAL int32_t *__restrict__ ai = as_global(TK.ai);
+        const TR_GLOBAL int32_t *__restrict__ bp = as_global(TK.bp);
+        const TR_GLOBAL int32_t *__restrict__ bi = as_global(TK.bi);
         const int32_t ncols = TK.n;
-        const int32_t a0 = TK.ap[r], a1 = TK.ap[r + 1];
+        const int32_t a0 = as_global(TK.ap)[r], a1 = as_global(TK.ap)[r + 1];
         int32_t row_total = 0;
         for (int64_t base = 0; base < ncols; base += (1 << PB)) {
             const int32_t width = (int32_t)min<int64_t>(1 << PB, (int64_t)ncols - base);
@@ -499,7 +501,7 @@ __global__ void k_item_cursor_len(const Item *__restrict__ items, const Counters
         if (I.wend - I.wbeg > 1) {
             const TaskDev &TK = tasks[task_of(tasks, nt, I.row)];
             const int32_t r = I.row - TK.row_base;
-            v = TK.ap[r + 1] - TK.ap[r];
+            v = as_global(TK.ap)[r + 1] - as_global(TK.ap)[r];
             if (v <= reg_max) v = 0;
         }
     }
@@ -618,14 +620,14 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
         const int32_t g = rows[li];
         const TaskDev &TK = tasks[task_of(tasks, nt, g)];
         const int32_t r = g - TK.row_base;
-        const int32_t *__restrict__ ai = TK.ai;
-        const double *__restrict__ av = TK.av;
-        const int32_t *__restrict__ bp = TK.bp;
-        const int32_t *__restrict__ bi = TK.bi;
-        const double *__restrict__ bv = TK.bv;
+        const TR_GLOBAL int32_t *__restrict__ ai = as_global(TK.ai);
+        const TR_GLOBAL double *__restrict__ av = as_global(TK.av);
+        const TR_GLOBAL int32_t *__restrict__ bp = as_global(TK.bp);
+        const TR_GLOBAL int32_t *__restrict__ bi = as_global(TK.bi);
+        const TR_GLOBAL double *__restrict__ bv = as_global(TK.bv);
         for (int t = l; t < T; t += kWave) keys[t] = kEmpty;
         wave_sync();
-        const int32_t a0 = TK.ap[r], a1 = TK.ap[r + 1];
+        const int32_t a0 = as_global(TK.ap)[r], a1 = as_global(TK.ap)[r + 1];
         for (int32_t pb = a0; pb < a1; pb += kWave) {
             int32_t bs = 0, be = 0;
             double a = 0.0;
@@ -689,8 +691,8 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
         if (l == 0 && n != expect) atomicOr(&cnt->overflow, 2);
         const int nw = n < expect ? n : expect;  // never write past the row
         const Panel &P = find_panel(panels, npanels, g);
-        int32_t *oi = P.idx + (rs - P.nnz_begin);
-        double *ov = P.val + (rs - P.nnz_begin);
+        TR_GLOBAL int32_t *oi = as_global(P.idx) + (rs - P.nnz_begin);
+        TR_GLOBAL double *ov = as_global(P.val) + (rs - P.nnz_begin);
         for (int t = l; t < nw; t += kWave) {
             oi[t] = keys[t];
             ov[t] = vals[t];
@@ -792,18 +794,18 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
         const int32_t g = I.row;
         const TaskDev &TK = tasks[task_of(tasks, nt, g)];
         const int32_t r = g - TK.row_base;
-        const int32_t *__restrict__ ai = TK.ai;
-        const double *__restrict__ av = TK.av;
-        const int32_t *__restrict__ bp = TK.bp;
-        const int32_t *__restrict__ bi = TK.bi;
-        const double *__restrict__ bv = TK.bv;
+        const TR_GLOBAL int32_t *__restrict__ ai = as_global(TK.ai);
+        const TR_GLOBAL double *__restrict__ av = as_global(TK.av);
+        const TR_GLOBAL int32_t *__restrict__ bp = as_global(TK.bp);
+        const TR_GLOBAL int32_t *__restrict__ bi = as_global(TK.bi);
+        const TR_GLOBAL double *__restrict__ bv = as_global(TK.bv);
         const int32_t ncols = TK.n;
-        const int32_t a0 = TK.ap[r], a1 = TK.ap[r + 1];
+        const int32_t a0 = as_global(TK.ap)[r], a1 = as_global(TK.ap)[r + 1];
         int2 *cur = cursors + cursor_base[itx];
         const int64_t rs = rowptr[g];
         const Panel &P = find_panel(panels, npanels, g);
-        int32_t *oi_row = P.idx + (rs - P.nnz_begin);
-        double *ov_row = P.val + (rs - P.nnz_begin);
+        TR_GLOBAL int32_t *oi_row = as_global(P.idx) + (rs - P.nnz_begin);
+        TR_GLOBAL double *ov_row = as_global(P.val) + (rs - P.nnz_begin);
         // rows with <= NT * kRegSlots entries: the A entries and B ranges of
         // thread tid (slots tid, tid + NT, ...) and their cursors live in
         // registers; longer rows keep cursors in the item's cursor array
@@ -990,8 +992,8 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             // entries created in this window (each thread counted its fresh inserts)
             const int32_t wf = wave_sum(nfresh);
             if (l == 0 && wf) atomicAdd(&L.nocc, wf);
-            int32_t *oi = oi_row + W.off;
-            double *ov = ov_row + W.off;
+            TR_GLOBAL int32_t *oi = oi_row + W.off;
+            TR_GLOBAL double *ov = ov_row + W.off;
             const int nwords = dense ? (W.c1 - W.c0 + 63) >> 6 : kBmWords;
             // each thread's entries: rank << 20 | column offset (kNone: no
             // entry) and value
