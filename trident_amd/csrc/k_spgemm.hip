@@ -730,7 +730,38 @@ struct HeavyWgLds {
     int32_t nocc;
     double pa[NT], pb[NT];  // the step's products (A value, B value) by thread
     uint16_t link[NT];      // next older product of the same slot in this step
+    // segment starts of three rotating product steps: bit p of smask[b] is
+    // set where a segment starts at step position p (position 0 always holds
+    // the segment that covers the step's first product), smark[b][p] is
+    // that segment's index
+    unsigned long long smask[3][NT / kWave];
+    typename std::conditional<(NT <= 256), uint8_t, uint16_t>::type smark[3][NT];
 };
+
+// marks the segment [excl, excl + len) of the calling thread in the step that
+// starts at product `base`
+template <int NT, typename M>
+__device__ __forceinline__ void mark_segment(unsigned long long *smask, M *smark, int32_t base, int32_t excl,
+                                             int32_t len) {
+    if (len <= 0) return;
+    const int32_t rel = excl - base;
+    int32_t pos = -1;
+    if (rel >= 0 && rel < NT) pos = rel;
+    else if (rel < 0 && excl + len > base) pos = 0;
+    if (pos >= 0) {
+        smark[pos] = (M)threadIdx.x;
+        atomicOr(&smask[pos >> 6], 1ull << (pos & 63));
+    }
+}
+
+// segment of step position rel: the mark at the highest set bit <= rel
+template <typename M>
+__device__ __forceinline__ int owner_marked(const unsigned long long *smask, const M *smark, int rel) {
+    int w = rel >> 6;
+    unsigned long long m = smask[w] & (~0ull >> (63 - (rel & 63)));
+    while (m == 0ull && w > 0) m = smask[--w];
+    return (int)smark[w * 64 + 63 - __clzll((long long)m)];
+}
 
 // workgroup exclusive scan of v; returns the exclusive prefix, *total = sum
 template <int NT>
@@ -748,17 +779,6 @@ __device__ __forceinline__ int32_t wg_excl_scan(int32_t v, int32_t *wsum, int32_
     }
     *total = tot;
     return off + inc - v;
-}
-
-// largest o in [0, n) with x[o] <= key (x nondecreasing, x[0] == 0 <= key)
-__device__ __forceinline__ int owner_lds(const int32_t *x, int n, int32_t key) {
-    int lo = 0, hi = n;  // x[lo] <= key < x[hi] (x[n] treated as +inf)
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (x[mid] <= key) lo = mid;
-        else hi = mid;
-    }
-    return lo;
 }
 
 template <int T, int NT>
@@ -837,6 +857,7 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             const bool dense = (W.c1 - W.c0) <= kWinMaxWidth;
             int nfresh = 0;
             for (int32_t pb = a0, slot = 0; pb < a1; pb += NT, ++slot) {
+                if (tid < 2 * (NT / kWave)) (&L.smask[0][0])[tid] = 0ull;
                 // this thread's k-slot: A entry p, B segment [s, e) inside the window
                 const int32_t p = pb + tid;
                 int32_t s = 0, e = 0;
@@ -892,23 +913,31 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                     }
                 }
                 int32_t total;
-                const int32_t excl = wg_excl_scan<NT>(e - s, L.wsum, &total);
+                const int32_t slen = e - s;
+                const int32_t excl = wg_excl_scan<NT>(slen, L.wsum, &total);
                 L.seg_s[tid] = s;
                 L.seg_x[tid] = excl;
                 L.seg_a[tid] = a;
-                const int nk = min(NT, a1 - pb);
+                // segment starts of the first two steps (their masks were
+                // cleared before the scan's barrier)
+                mark_segment<NT>(L.smask[0], L.smark[0], 0, excl, slen);
+                mark_segment<NT>(L.smask[1], L.smark[1], NT, excl, slen);
                 __syncthreads();
                 // products x = x0 + tid, flattened in A-order; the B entry of the
                 // next step is loaded while the current step is applied
                 int32_t jn = 0;
                 double bn = 0.0, an = 0.0;
                 if (tid < total) {
-                    const int o = owner_lds(L.seg_x, nk, tid);
+                    const int o = owner_marked(L.smask[0], L.smark[0], tid);
                     const int32_t q = L.seg_s[o] + (tid - L.seg_x[o]);
                     an = L.seg_a[o];
                     jn = bi[q];
                     bn = bv[q];
                 }
+                // step k reads the marks of step k + 1 (buffer bA), clears
+                // buffer bC before its first barrier and marks step k + 2 in it
+                // between its barriers
+                int bA = 1, bC = 2;
                 for (int32_t x0 = 0; x0 < total; x0 += NT) {
                     const int32_t x = x0 + tid;
                     bool act = x < total;
@@ -916,12 +945,13 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                     const double b = bn, ao = an;
                     const int32_t xn = x + NT;
                     if (xn < total) {
-                        const int o = owner_lds(L.seg_x, nk, xn);
+                        const int o = owner_marked(L.smask[bA], L.smark[bA], tid);
                         const int32_t q = L.seg_s[o] + (xn - L.seg_x[o]);
                         an = L.seg_a[o];
                         jn = bi[q];
                         bn = bv[q];
                     }
+                    if (tid < NT / kWave) L.smask[bC][tid] = 0ull;
                     // insert: the first product of a slot in this step creates or
                     // claims it; every later one pushes itself onto the slot's
                     // list (the tag field holds the last pusher, link[] the rest)
@@ -984,7 +1014,10 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                         L.vals[h] = v;
                         L.keytag[h] = key | kNoTag;
                     }
+                    if (x0 + 2 * NT < total) mark_segment<NT>(L.smask[bC], L.smark[bC], x0 + 2 * NT, excl, slen);
                     __syncthreads();
+                    bA = bC;
+                    bC = bC == 2 ? 0 : bC + 1;
                 }
                 __syncthreads();  // seg_* are rewritten by the next batch
             }
