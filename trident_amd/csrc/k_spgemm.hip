@@ -961,28 +961,25 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                         key = off << kTagBits;
                         const uint32_t me = key | (uint32_t)tid;
                         h = hash_slot((int32_t)off, kLog);
-                        uint32_t prev = kNoTag;
-                        bool placed = false;
-                        for (int probe = 0; probe < T; ++probe) {
-                            const uint32_t old = atomicCAS(&L.keytag[h], kSlotFree, me);
-                            if (old == kSlotFree) {
-                                L.vals[h] = 0.0;
-                                if (dense) atomicOr(&L.bm[off >> 6], 1ull << (off & 63));
-                                ++nfresh;
-                                placed = true;
-                                break;
-                            }
-                            if ((old >> kTagBits) == off) {
-                                prev = atomicExch(&L.keytag[h], me) & kNoTag;
-                                placed = true;
-                                break;
-                            }
+                        // probe to a free slot (claimed) or this key's slot; the
+                        // bound only guards a table the symbolic counts overfilled
+                        uint32_t old = atomicCAS(&L.keytag[h], kSlotFree, me);
+                        for (int probe = 1; old != kSlotFree && (old >> kTagBits) != off && probe < T; ++probe) {
                             h = (h + 1) & (T - 1);
+                            old = atomicCAS(&L.keytag[h], kSlotFree, me);
                         }
-                        if (!placed) {
+                        uint32_t prev = kNoTag;
+                        if (old == kSlotFree) {
+                            L.vals[h] = 0.0;
+                            if (dense) atomicOr(&L.bm[off >> 6], 1ull << (off & 63));
+                            ++nfresh;
+                        } else if ((old >> kTagBits) == off) {
+                            prev = atomicExch(&L.keytag[h], me) & kNoTag;
+                        } else {
                             atomicOr(&cnt->overflow, 8);  // table full: symbolic counts were wrong
                             act = false;
-                        } else {
+                        }
+                        if (act) {
                             L.link[tid] = (uint16_t)prev;
                             L.pa[tid] = ao;
                             L.pb[tid] = b;
