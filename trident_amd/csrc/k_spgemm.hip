@@ -713,6 +713,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
 // A-order) through atomicMin rounds.  A presence bitmap over the window gives
 // every entry its sorted rank (prefix popcounts) for the write-out.
 constexpr int kRegSlots = 4;  // register-held A entries per thread in the heavy numeric kernel
+constexpr int kSmallNT = 256;  // threads of the 2048-slot heavy numeric
 constexpr uint32_t kSlotFree = 0xffffffffu;
 constexpr int kTagBits = 10;                        // thread ids < 1023
 constexpr uint32_t kNoTag = (1u << kTagBits) - 1u;  // slot key = column offset << kTagBits | tag
@@ -723,9 +724,7 @@ struct HeavyWgLds {
     double vals[T];
     unsigned long long bm[kBmWords];
     uint16_t wpre[kBmWords];
-    int32_t seg_s[NT];
-    int32_t seg_x[NT + 1];
-    double seg_a[NT];
+    int32_t seg_d[NT];  // B position minus flattened product position of segment (A entry) o
     int32_t wsum[NT / kWave];
     int32_t nocc;
     double pa[NT], pb[NT];  // the step's products (A value, B value) by thread
@@ -826,21 +825,18 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
         const Panel &P = find_panel(panels, npanels, g);
         TR_GLOBAL int32_t *oi_row = as_global(P.idx) + (rs - P.nnz_begin);
         TR_GLOBAL double *ov_row = as_global(P.val) + (rs - P.nnz_begin);
-        // rows with <= NT * kRegSlots entries: the A entries and B ranges of
-        // thread tid (slots tid, tid + NT, ...) and their cursors live in
+        // rows with <= NT * kRegSlots entries: the B ranges of thread tid's A
+        // entries (slots tid, tid + NT, ...) and their cursors live in
         // registers; longer rows keep cursors in the item's cursor array
         const bool reg_row = (a1 - a0) <= NT * kRegSlots;
         // (before the item's first window the cursor holds the start of B_k)
         int32_t rk_be[kRegSlots], rk_cur[kRegSlots], rk_nxt[kRegSlots];
-        double rk_a[kRegSlots];
 #pragma unroll
         for (int u = 0; u < kRegSlots; ++u) {
             rk_be[u] = rk_cur[u] = rk_nxt[u] = 0;
-            rk_a[u] = 0.0;
             const int32_t p = a0 + u * NT + tid;
             if (reg_row && p < a1) {
                 const int32_t k = ai[p];
-                rk_a[u] = av[p];
                 rk_cur[u] = bp[k];
                 rk_be[u] = bp[k + 1];
             }
@@ -861,7 +857,6 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                 // this thread's k-slot: A entry p, B segment [s, e) inside the window
                 const int32_t p = pb + tid;
                 int32_t s = 0, e = 0;
-                double a = 0.0;
                 if (p < a1) {
                     // c: cursor (first entry at or after this window's start),
                     // v: its column (INT32_MAX at the end of B_k); known once the
@@ -873,14 +868,12 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                         for (int u = 0; u < kRegSlots; ++u)
                             if (u == slot) {
                                 be = rk_be[u];
-                                a = rk_a[u];
                                 c = rk_cur[u];
                                 v = rk_nxt[u];
                             }
                         bs = c;
                     } else {
                         const int32_t k = ai[p];
-                        a = av[p];
                         bs = bp[k];
                         be = bp[k + 1];
                         if (have_cursor && !full) {
@@ -915,9 +908,7 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                 int32_t total;
                 const int32_t slen = e - s;
                 const int32_t excl = wg_excl_scan<NT>(slen, L.wsum, &total);
-                L.seg_s[tid] = s;
-                L.seg_x[tid] = excl;
-                L.seg_a[tid] = a;
+                L.seg_d[tid] = s - excl;
                 // segment starts of the first two steps (their masks were
                 // cleared before the scan's barrier)
                 mark_segment<NT>(L.smask[0], L.smark[0], 0, excl, slen);
@@ -929,8 +920,8 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                 double bn = 0.0, an = 0.0;
                 if (tid < total) {
                     const int o = owner_marked(L.smask[0], L.smark[0], tid);
-                    const int32_t q = L.seg_s[o] + (tid - L.seg_x[o]);
-                    an = L.seg_a[o];
+                    const int32_t q = L.seg_d[o] + tid;
+                    an = av[pb + o];
                     jn = bi[q];
                     bn = bv[q];
                 }
@@ -946,8 +937,8 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                     const int32_t xn = x + NT;
                     if (xn < total) {
                         const int o = owner_marked(L.smask[bA], L.smark[bA], tid);
-                        const int32_t q = L.seg_s[o] + (xn - L.seg_x[o]);
-                        an = L.seg_a[o];
+                        const int32_t q = L.seg_d[o] + xn;
+                        an = av[pb + o];
                         jn = bi[q];
                         bn = bv[q];
                     }
@@ -1025,16 +1016,23 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             TR_GLOBAL int32_t *oi = oi_row + W.off;
             TR_GLOBAL double *ov = ov_row + W.off;
             const int nwords = dense ? (W.c1 - W.c0 + 63) >> 6 : kBmWords;
-            // each thread's entries: rank << 20 | column offset (kNone: no
-            // entry) and value
-            constexpr int kSt = T / NT;
-            static_assert(kSt * NT >= kSparseH + 127, "a sparse window must fit the per-thread stash");
+            // Each thread's entries as rank << 20 | column offset (kNone: no
+            // entry) and value.  The table's lower half [0, T / 2) is also the
+            // staging area (a window holds at most T / 2 entries): its slots
+            // are stashed in registers before a barrier, the upper half is
+            // read and staged after it.
+            constexpr int kSt = (T + NT - 1) / NT;
+            constexpr int kLo = (T / 2 + NT - 1) / NT;
+            constexpr int kSp = (kSparseH + 127 + NT - 1) / NT;  // sparse entries per thread
+            constexpr int kStash = kLo > kSp ? kLo : kSp;
             constexpr uint32_t kNone = 0xffffffffu;
-            uint32_t st_rk[kSt];
-            double st_v[kSt];
+            uint32_t st_rk[kStash];
+            double st_v[kStash];
+            const int32_t lim = min(W.cnt, T / 2);  // never write past the window
+            int32_t *sidx = reinterpret_cast<int32_t *>(L.keytag);
             if (dense) {
                 // prefix popcounts of the presence words
-                constexpr int kPer = kBmWords / NT > 0 ? kBmWords / NT : 1;
+                constexpr int kPer = (kBmWords + NT - 1) / NT;
                 int32_t wc[kPer];
                 int32_t tsum = 0;
 #pragma unroll
@@ -1054,16 +1052,33 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                 __syncthreads();
                 if (tid == 0 && L.nocc != W.cnt) atomicOr(&cnt->overflow, 4);
 #pragma unroll
-                for (int u = 0; u < kSt; ++u) {
+                for (int u = 0; u < kStash; ++u) {
                     const int h = tid + u * NT;
-                    const uint32_t kt = L.keytag[h];
                     st_rk[u] = kNone;
+                    if (u >= kLo || h >= T / 2) continue;
+                    const uint32_t kt = L.keytag[h];
                     if (kt == kSlotFree) continue;
                     const uint32_t off = kt >> kTagBits;
                     const int wd = (int)(off >> 6);
                     const uint32_t rr = (uint32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
                     st_rk[u] = (rr << 20) | off;
                     st_v[u] = L.vals[h];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < kSt; ++u) {
+                    const int h = tid + u * NT;
+                    if (h < T / 2 || h >= T) continue;
+                    const uint32_t kt = L.keytag[h];
+                    if (kt == kSlotFree) continue;
+                    const uint32_t off = kt >> kTagBits;
+                    const int wd = (int)(off >> 6);
+                    const int32_t rr = (int32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
+                    if (rr < lim) {
+                        const double v = L.vals[h];
+                        sidx[rr] = W.c0 + (int32_t)off;
+                        L.vals[rr] = v;
+                    }
                 }
             } else {
                 // compact the occupied slots (keys into the bitmap's space,
@@ -1087,25 +1102,24 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                 __syncthreads();
                 if (tid == 0 && (L.nocc != W.cnt || !fits)) atomicOr(&cnt->overflow, 4);
 #pragma unroll
-                for (int u = 0; u < kSt; ++u) {
+                for (int u = 0; u < kStash; ++u) {
                     const int e = tid + u * NT;
                     st_rk[u] = kNone;
-                    if (!fits || e >= tot) continue;
+                    if (u >= kSp || !fits || e >= tot) continue;
                     const uint32_t k = lkey[e];
                     uint32_t rr = 0;
                     for (int q = 0; q < tot; ++q) rr += lkey[q] < k ? 1u : 0u;
                     st_rk[u] = (rr << 20) | k;
                     st_v[u] = L.vals[lslot[e]];
                 }
+                __syncthreads();
             }
-            // stage the window's entries in rank order in the table's space,
-            // then write them out with coalesced stores
-            __syncthreads();
-            int32_t *sidx = reinterpret_cast<int32_t *>(L.keytag);
+            // stage the stashed entries in rank order, then write the window
+            // out with coalesced stores
 #pragma unroll
-            for (int u = 0; u < kSt; ++u) {
+            for (int u = 0; u < kStash; ++u) {
                 const int32_t rr = (int32_t)(st_rk[u] >> 20);
-                if (st_rk[u] != kNone && rr < W.cnt) {  // never write past the window
+                if (st_rk[u] != kNone && rr < lim) {
                     sidx[rr] = W.c0 + (int32_t)(st_rk[u] & 0xfffffu);
                     L.vals[rr] = st_v[u];
                 }
@@ -1262,7 +1276,7 @@ int64_t mul_cursor_prep(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool capture) {
     }
     if (ws.owned && !capture) pl.scan_tmp = (int64_t *)ws.get(S_SCAN_TMP, scan_tmp_bytes(std::max<int64_t>(pl.item_cap, pl.rows_total)));
     TR_LAUNCH(ctx, "item_cursor_len", k_item_cursor_len, div_up((uint64_t)pl.item_cap, 256), 256, 0, pl.items, pl.cnt,
-              pl.d_tasks, pl.nt, pl.item_cap, (pl.tune.heavy_table == 4096 ? 512 : 256) * kRegSlots, pl.clen);
+              pl.d_tasks, pl.nt, pl.item_cap, (pl.tune.heavy_table == 4096 ? 512 : kSmallNT) * kRegSlots, pl.clen);
     // column order of the items
     if (!capture) {
         pl.order = (int32_t *)ws.get(S_ORDER, sizeof(int32_t) * ((size_t)pl.item_cap + 1));
@@ -1298,7 +1312,7 @@ void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *dp, int np) {
             TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<4096, 512>), grid, 512, 0, pl.d_tasks, pl.nt, pl.items,
                       pl.cnt, pl.wins, pl.order, pl.cbase, pl.cursors, pl.rowptr, dp, np);
         else
-            TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<2048, 256>), grid, 256, 0, pl.d_tasks, pl.nt, pl.items,
+            TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<2048, kSmallNT>), grid, kSmallNT, 0, pl.d_tasks, pl.nt, pl.items,
                       pl.cnt, pl.wins, pl.order, pl.cbase, pl.cursors, pl.rowptr, dp, np);
     }
 }
