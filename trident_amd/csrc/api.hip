@@ -21,11 +21,39 @@ static size_t round_bytes(size_t b) {
 
 Pool::~Pool() {
     for (auto &kv : free_) (void)hipFree(kv.second);
-    for (auto &kv : live_) (void)hipFree(kv.first);
+    for (auto &kv : live_)
+        if (!in_arena(kv.first)) (void)hipFree(kv.first);
+    if (base_) (void)hipFree(base_);
+}
+
+bool Pool::reserve(size_t bytes) {
+    if (base_ || bytes == 0) return base_ != nullptr;
+    const size_t b = round_bytes(bytes);
+    if (hipMalloc(&base_, b) != hipSuccess) {
+        (void)hipGetLastError();
+        base_ = nullptr;
+        return false;
+    }
+    ++n_malloc;
+    arena_.init(b);
+    return true;
 }
 
 void *Pool::alloc(size_t bytes) {
     size_t b = round_bytes(bytes);
+    if (base_) {
+        size_t off = 0;
+        if (arena_.alloc(b, off)) {
+            void *p = (char *)base_ + off;
+            if (limit_ && in_use_ + b > limit_) {
+                arena_.free(off, b);
+                fail(TR_ERR_OOM, "device pool limit reached");
+            }
+            live_[p] = b;
+            in_use_ += b;
+            return p;
+        }
+    }
     auto it = free_.lower_bound(b);
     if (it != free_.end() && it->first <= 2 * b + (4u << 20)) {
         void *p = it->second;
@@ -37,6 +65,7 @@ void *Pool::alloc(size_t bytes) {
         return p;
     }
     if (limit_ && in_use_ + b > limit_) fail(TR_ERR_OOM, "device pool limit reached");
+    if (cache_cap_ && in_use_ + cached_ + b > cache_cap_) trim_to(cache_cap_ > b ? cache_cap_ - b : 0);
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, b);
     if (e != hipSuccess) {
@@ -61,6 +90,10 @@ void Pool::release(void *p) {
     size_t sz = it->second;
     live_.erase(it);
     in_use_ -= sz;
+    if (in_arena(p)) {
+        arena_.free((size_t)((char *)p - (char *)base_), sz);
+        return;
+    }
     free_.emplace(sz, p);
     cached_ += sz;
 }
@@ -75,6 +108,16 @@ void Pool::trim() {
     for (auto &kv : free_) (void)hipFree(kv.second);
     free_.clear();
     cached_ = 0;
+}
+
+void Pool::trim_to(size_t total) {
+    while (!free_.empty() && in_use_ + cached_ > total) {
+        auto it = std::prev(free_.end());
+        (void)hipFree(it->second);
+        ++n_free;
+        cached_ -= it->first;
+        free_.erase(it);
+    }
 }
 
 // ------------------------------------------------------------------ scratch / staging

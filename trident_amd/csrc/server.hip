@@ -60,9 +60,12 @@ struct Request {
 
 // A finished batch; stays alive while a result or partial refers to it.  Its
 // admitted memory bound is returned to the budget when it is destroyed.
-// The worker's pool keeps freed blocks for reuse (allocation and release of
-// large blocks are expensive and releases synchronise the device); beyond its
-// share of the budget it returns them.  An allocation that still fails trims
+// Each worker's pool reserves the worker's share of the budget once and
+// places every batch's blocks in it (tr::Arena): a hipMalloc larger than
+// what the runtime has cached from earlier frees takes about 14 ms per GB
+// with the GPU idle, and batch sizes change all the time.  Blocks that do not
+// fit the reservation come from the pool's cache, whose size (in use +
+// cached) stays within the share.  An allocation that still fails trims
 // every worker's pool and retries (worker_loop).
 struct Batch {
     tr_batch_t b = nullptr;
@@ -75,8 +78,7 @@ struct Batch {
         if (b) {
             std::lock_guard<std::mutex> lk(*ctx_mu);
             tr_batch_destroy(b);
-            size_t used = 0, cached = 0;
-            if (tr_pool_stats(ctx, &used, &cached) == TR_OK && cached > cache_cap) (void)tr_pool_trim(ctx);
+            ctx->pool.trim_to(cache_cap);
         }
         if (in_use) in_use->fetch_sub(bound);
     }
@@ -243,12 +245,32 @@ void worker_loop(tr_server *s, Worker *w) {
         const auto tb0 = Clock::now();
         const size_t nm0 = w->ctx->pool.n_malloc, nf0 = w->ctx->pool.n_free;
 #endif
+#ifdef TR_SERVER_TRACE
+        double t_lock = 0, t_create = 0, t_sync = 0;
+        size_t mem_free = 0, mem_total = 0;
+        (void)hipMemGetInfo(&mem_free, &mem_total);
+#endif
         for (int attempt = 0; attempt < 2; ++attempt) {
             {
+#ifdef TR_SERVER_TRACE
+                const auto ta = Clock::now();
+#endif
                 std::lock_guard<std::mutex> ck(w->mu);
+#ifdef TR_SERVER_TRACE
+                const auto tb = Clock::now();
+#endif
                 rc = tr::batch_create(w->ctx, (int32_t)take.size(), As.data(), Bs.data(), nullptr,
                                       TR_BATCH_NO_GRAPH | tr::kBatchSingleRun, false, &bt->b);
+#ifdef TR_SERVER_TRACE
+                const auto tc = Clock::now();
+#endif
                 if (rc == TR_OK) rc = tr_ctx_synchronize(w->ctx);
+#ifdef TR_SERVER_TRACE
+                const auto td = Clock::now();
+                t_lock += std::chrono::duration<double, std::milli>(tb - ta).count();
+                t_create += std::chrono::duration<double, std::milli>(tc - tb).count();
+                t_sync += std::chrono::duration<double, std::milli>(td - tc).count();
+#endif
                 if (rc == TR_OK) tr_batch_stats(bt->b, &fl, nullptr, nullptr);
                 if (rc != TR_OK) err = tr_last_error(w->ctx);
             }
@@ -260,10 +282,13 @@ void worker_loop(tr_server *s, Worker *w) {
             }
         }
 #ifdef TR_SERVER_TRACE
-        fprintf(stderr, "[srv] worker %p tasks %zu bound %.1f MB flops %.3e batch %.2f ms mallocs %zu frees %zu rc %d\n",
+        fprintf(stderr,
+                "[srv] worker %p tasks %zu bound %.1f MB flops %.3e batch %.2f ms (lock %.2f create %.2f sync %.2f) "
+                "mallocs %zu frees %zu rc %d free %.1f GB pool %.1f+%.1f GB\n",
                 (void *)w, take.size(), admitted / 1e6, (double)fl,
-                std::chrono::duration<double, std::milli>(Clock::now() - tb0).count(),
-                w->ctx->pool.n_malloc - nm0, w->ctx->pool.n_free - nf0, rc);
+                std::chrono::duration<double, std::milli>(Clock::now() - tb0).count(), t_lock, t_create, t_sync,
+                w->ctx->pool.n_malloc - nm0, w->ctx->pool.n_free - nf0, rc, mem_free / 1e9,
+                w->ctx->pool.in_use() / 1e9, w->ctx->pool.cached() / 1e9);
 #endif
         std::lock_guard<std::mutex> lk(s->mu);
         ++s->batches;
@@ -298,6 +323,10 @@ int tr_server_create(int device, size_t mem_budget, int32_t workers, int32_t max
             delete s;
             return rc;
         }
+        // the worker's share of the budget, reserved once; blocks beyond it
+        // (if the reservation fails or is fragmented) come from the cache
+        (void)w->ctx->pool.reserve(mem_budget / (size_t)workers);
+        w->ctx->pool.set_cache_cap(mem_budget / (size_t)workers);
         s->workers.push_back(std::move(w));
     }
     for (auto &w : s->workers) w->th = std::thread(worker_loop, s, w.get());

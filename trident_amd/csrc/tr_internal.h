@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/trident_hip.h"
+#include "arena.h"
 
 namespace tr {
 
@@ -49,15 +50,29 @@ class Pool {
     bool owns(void *p) const { return live_.count(p) != 0; }
     size_t size_of(void *p) const;
     void trim();
+    void trim_to(size_t total);  // frees cached blocks, largest first, until in use + cached <= total
     size_t in_use() const { return in_use_; }
     size_t cached() const { return cached_; }
     size_t n_malloc = 0, n_free = 0;  // device allocations / releases made (not cache hits)
     void set_limit(size_t bytes) { limit_ = bytes; }
+    // Before a new device allocation, cached blocks are freed (largest first)
+    // until in use + cached + the new block fit `bytes` (0: no cap).  Idle
+    // cached blocks still count against the device memory the driver places.
+    void set_cache_cap(size_t bytes) { cache_cap_ = bytes; }
+    // Reserves `bytes` of device memory once; later blocks are placed in it
+    // (best fit, coalescing) before the cache and hipMalloc are tried.
+    // Returns false if the reservation cannot be allocated.
+    bool reserve(size_t bytes);
 
   private:
+    bool in_arena(const void *p) const {
+        return base_ && (const char *)p >= (const char *)base_ && (const char *)p < (const char *)base_ + arena_.size();
+    }
     std::multimap<size_t, void *> free_;
     std::unordered_map<void *, size_t> live_;
-    size_t in_use_ = 0, cached_ = 0, limit_ = 0;
+    size_t in_use_ = 0, cached_ = 0, limit_ = 0, cache_cap_ = 0;
+    void *base_ = nullptr;
+    Arena arena_;
 };
 
 struct TimingRecord {
