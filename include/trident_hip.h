@@ -198,9 +198,11 @@ int tr_batch_destroy(tr_batch_t batch);
  * steps that arrive while a batch runs join the next one.  One worker is
  * the efficient choice on one GPU: batches of several workers overlap on
  * the device, but concurrent products compete for its caches (measured
- * 3.4x lower throughput with two workers on MI355X).  Results are
- * bit-identical to the same product run alone.  No counterpart in the
- * reference, which has no serving layer. */
+ * 3.4x lower throughput with two workers on MI355X).  Each worker reserves
+ * mem_budget / workers of device memory when the server is created and
+ * places its batches' blocks in it, so a running server makes no device
+ * allocation.  Results are bit-identical to the same product run alone.
+ * No counterpart in the reference, which has no serving layer. */
 #define TR_REQ_SPGEMM 0
 #define TR_REQ_GALERKIN 1
 #define TR_REQ_DISCARD 0x100  /* or-ed into the kind: drop the result when done */
