@@ -111,6 +111,15 @@ def download_rows(ctx, panels, rb, hptrs, rows):
     return out
 
 
+def _cpu_model() -> str:
+    """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
+    try:
+        names = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs visible)" if names else "unknown"
+    except OSError:
+        return "unknown"
+
+
 def cpu_baseline_and_check(A, f, ctx, panels, rb, hptrs, target_mults, seed=7):
     """Time the oracle on a random row sample of ~target_mults multiplications
     and compare those rows of C with the GPU result."""
@@ -147,6 +156,7 @@ def cpu_baseline_and_check(A, f, ctx, panels, rb, hptrs, target_mults, seed=7):
         "value": round(2.0 * mults / t_cpu / 1e9, 3),
         "unit": "GFLOP/s",
         "cores": oracle.num_threads(),
+        "cpu": _cpu_model(),
         "kind": "port",
         "sample": (f"{len(rows)} random rows of A (seed {seed}) times A: {mults:.3e} multiplications, "
                    f"{int(cp[-1]):.3e} outputs, {t_cpu:.2f} s; the repository's FP64 OpenMP Gustavson oracle "
