@@ -796,17 +796,7 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
     if (tid == 0) L.nocc = 0;
     __syncthreads();
     const int32_t nitems = cnt->items;
-#if TR_XCD_MAP
-    // Workgroups are dispatched round-robin over the 8 XCDs: runs of
-    // kXcdRun consecutive items (mostly windows of one row, which read the
-    // same B rows) go to one XCD and the runs rotate over the XCDs.
-    // gridDim.x is a multiple of 8 * kXcdRun.
-    constexpr int32_t kXcdRun = 64;
-    const int32_t r = (int32_t)(blockIdx.x >> 3), x = (int32_t)(blockIdx.x & 7);
-    const int32_t slot = ((r / kXcdRun) * 8 + x) * kXcdRun + r % kXcdRun;
-#else
     const int32_t slot = (int32_t)blockIdx.x;
-#endif
     for (int32_t ord = slot; ord < nitems; ord += gridDim.x) {
         const int32_t itx = order[ord];
         const Item I = items[itx];
@@ -1175,8 +1165,6 @@ uint32_t cap_grid(int64_t need, int64_t cap) {
     return (uint32_t)std::min<int64_t>(need, cap);
 }
 
-// heavy numeric grid: a multiple of the XCD run pattern when items are mapped to XCDs
-uint32_t heavy_grid(uint32_t g) { return TR_XCD_MAP ? (g + 511u) & ~511u : g; }
 
 template <int T>
 void launch_sym_light(tr_ctx *ctx, MulPlan &pl, int c) {
@@ -1307,7 +1295,7 @@ void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *dp, int np) {
     if (pl.h.items > 0) {
         // one workgroup per item, grid-stride beyond 2^22 workgroups (the
         // dispatch packet counts work-items in 32 bits)
-        const uint32_t grid = heavy_grid(cap_grid(pl.h.items, 1 << 22));
+        const uint32_t grid = cap_grid(pl.h.items, 1 << 22);
         if (pl.tune.heavy_table == 4096)
             TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<4096, 512>), grid, 512, 0, pl.d_tasks, pl.nt, pl.items,
                       pl.cnt, pl.wins, pl.order, pl.cbase, pl.cursors, pl.rowptr, dp, np);

@@ -7,13 +7,10 @@
 #include <cstdint>
 #include <vector>
 
-#ifndef TR_XCD_MAP
-#define TR_XCD_MAP 0
-#endif
 
 namespace tr {
 
-// tuning constants (see DESIGN notes in the repository documentation)
+// tuning constants
 constexpr int kLightMaxFCap = 1024; // largest light/heavy cut (light tables reach 2048 slots)
 constexpr int kNumLight = 6;       // light classes: T = 64 << c, c < kNumLight
 constexpr int kHeavyClass = kNumLight;      // heavy rows with more than kMidMaxF multiplications
