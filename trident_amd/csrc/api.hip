@@ -331,7 +331,7 @@ static int guarded(tr_ctx *ctx, F &&body) {
 extern "C" {
 
 const char *tr_version(void) {
-    return "trident_hip gfx950: light<=1024 flops wave-hash, heavy bitmap+windows (H=384, T=1024), merge-path add";
+    return "trident_hip gfx950: light rows wave hash, heavy rows bitmap windows + 2048-slot workgroup tables, merge-path add";
 }
 
 int tr_ctx_create(tr_ctx_t *out, int device, void *hip_stream) {
