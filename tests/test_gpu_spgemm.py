@@ -161,6 +161,33 @@ def test_heavy_sparse_rows(ctx, ncols):
     run(ctx, A, B, f"sparse heavy {ncols}")
 
 
+def test_galerkin_operators(ctx):
+    # the AMG products A*P and P^T*(A*P): light rows whose B rows hold about
+    # 4.5 entries, multiplied with their products flattened over the lanes
+    n = 20
+    A, P = gen.laplace27(n), gen.sa_prolongator(n)
+    AP = run(ctx, A, P, "A*P")
+    run(ctx, gen.transpose(P), AP, "P^T*(A*P)")
+
+
+def test_light_short_rows_ordered(ctx):
+    # 64 A entries per row over B rows of 3 entries in 16 columns: every
+    # round of 64 flattened products meets a column about 12 times, and the
+    # values span 2^-20..2^20, so any application out of A-order changes bits
+    rng = np.random.default_rng(17)
+    k, ncols = 400, 16
+    bl = [np.sort(rng.choice(ncols, size=3, replace=False)) for _ in range(k)]
+    bptr = np.concatenate([[0], np.cumsum([len(x) for x in bl])])
+    bval = rng.uniform(-1, 1, bptr[-1]) * np.exp2(rng.integers(-20, 21, bptr[-1]))
+    B = gen.from_arrays(k, ncols, bptr, np.concatenate(bl), bval)
+    m = 2000
+    al = [np.sort(rng.choice(k, size=64, replace=False)) for _ in range(m)]
+    aptr = np.concatenate([[0], np.cumsum([len(x) for x in al])])
+    aval = rng.uniform(-1, 1, aptr[-1]) * np.exp2(rng.integers(-20, 21, aptr[-1]))
+    A = gen.from_arrays(m, k, aptr, np.concatenate(al), aval)
+    run(ctx, A, B, "short B rows")
+
+
 def _device_bytes_equal(ctx, pa, pb, nbytes, chunk=1 << 29):
     """Byte equality of two device ranges, copied to the host in chunks."""
     ha = np.empty(max(1, min(chunk, nbytes)), np.uint8)

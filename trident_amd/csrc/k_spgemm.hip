@@ -284,6 +284,18 @@ __global__ __launch_bounds__(256) void k_sym_light(const TaskDev *__restrict__ t
         wave_sync();
         const int32_t a0 = as_global(TK.ap)[r], a1 = as_global(TK.ap)[r + 1];
         int cntv = 0;
+        auto insert = [&](int32_t j) {
+            uint32_t h = hash_slot(j, kLog);
+            for (int probe = 0; probe < T; ++probe) {
+                const int32_t old = atomicCAS(&tab[h], kEmpty, j);
+                if (old == kEmpty) {
+                    ++cntv;
+                    break;
+                }
+                if (old == j) break;
+                h = (h + 1) & (T - 1);
+            }
+        };
         for (int32_t pb = a0; pb < a1; pb += kWave) {
             int32_t bs = 0, be = 0;
             if (pb + l < a1) {
@@ -292,20 +304,29 @@ __global__ __launch_bounds__(256) void k_sym_light(const TaskDev *__restrict__ t
                 be = bp[k + 1];
             }
             const int nk = min(kWave, a1 - pb);
-            for (int t = 0; t < nk; ++t) {
-                const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
-                stream_cols(bi, s, e, [&](int32_t j) {
-                    uint32_t h = hash_slot(j, kLog);
-                    for (int probe = 0; probe < T; ++probe) {
-                        const int32_t old = atomicCAS(&tab[h], kEmpty, j);
-                        if (old == kEmpty) {
-                            ++cntv;
-                            break;
-                        }
-                        if (old == j) break;
-                        h = (h + 1) & (T - 1);
-                    }
-                });
+            const int32_t len = be - bs;
+            const int32_t inc = wave_incl_scan(len);
+            const int32_t total = __shfl(inc, kWave - 1, kWave);
+            if (total >= 32 * nk) {
+                // long B rows: lanes stream one B row at a time
+                for (int t = 0; t < nk; ++t) {
+                    const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
+                    stream_cols(bi, s, e, insert);
+                }
+            } else {
+                // short B rows: the chunk's products flattened over the lanes
+                // (counting distinct columns does not depend on the order);
+                // product x belongs to the last lane whose prefix is <= x
+                const int32_t excl = inc - len;
+                for (int32_t x0 = 0; x0 < total; x0 += kWave) {
+                    const int32_t x = x0 + l;
+                    int t = 0;
+#pragma unroll
+                    for (int step = kWave / 2; step > 0; step >>= 1)
+                        if (__shfl(excl, t + step, kWave) <= x) t += step;
+                    const int32_t q = __shfl(bs, t, kWave) + (x - __shfl(excl, t, kWave));
+                    if (x < total) insert(bi[q]);
+                }
             }
         }
         cntv = wave_sum(cntv);
@@ -609,12 +630,20 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
                                                            Counters *__restrict__ cnt, int cls,
                                                            const int64_t *__restrict__ rowptr,
                                                            const Panel *__restrict__ panels, int npanels) {
+    // tables of <= 512 slots also flatten chunks of short B rows over the
+    // lanes; their per-slot owner words order the chunk's products
+    constexpr bool kFlat = T <= 512;
+    constexpr int kOwn = kFlat ? T : 1;
     __shared__ int32_t keys_s[WPB][T];
     __shared__ double vals_s[WPB][T];
+    __shared__ uint32_t own_s[WPB][kOwn];
     constexpr uint32_t kLog = __builtin_ctz(T);
     const int w = threadIdx.x / kWave, l = lane_id();
     int32_t *keys = keys_s[w];
     double *vals = vals_s[w];
+    uint32_t *own = own_s[w];
+    if (kFlat)
+        for (int t = l; t < kOwn; t += kWave) own[t] = kWave;
     const int32_t nrows = cnt->cls[cls];
     for (int32_t li = blockIdx.x * WPB + w; li < nrows; li += gridDim.x * WPB) {
         const int32_t g = rows[li];
@@ -638,6 +667,57 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
                 be = bp[k + 1];
             }
             const int nk = min(kWave, a1 - pb);
+            if (kFlat) {
+                const int32_t len = be - bs;
+                const int32_t inc = wave_incl_scan(len);
+                const int32_t total = __shfl(inc, kWave - 1, kWave);
+                if (total < 24 * nk) {
+                    // Short B rows: 64 products of the chunk per round, flattened
+                    // in A-order over the lanes (product x belongs to the last
+                    // lane whose prefix is <= x).  Products that meet in a slot
+                    // are applied in lane (= A) order: each pass, the lowest
+                    // pending lane of every slot applies its product.
+                    const int32_t excl = inc - len;
+                    for (int32_t x0 = 0; x0 < total; x0 += kWave) {
+                        const int32_t x = x0 + l;
+                        int t = 0;
+#pragma unroll
+                        for (int step = kWave / 2; step > 0; step >>= 1)
+                            if (__shfl(excl, t + step, kWave) <= x) t += step;
+                        const int32_t q = __shfl(bs, t, kWave) + (x - __shfl(excl, t, kWave));
+                        const double at = __shfl(a, t, kWave);
+                        bool pend = x < total;
+                        uint32_t h = 0;
+                        double b = 0.0;
+                        if (pend) {
+                            const int32_t j = bi[q];
+                            b = bv[q];
+                            h = hash_slot(j, kLog);
+                            for (int probe = 0; probe < T; ++probe) {
+                                const int32_t old = atomicCAS(&keys[h], kEmpty, j);
+                                if (old == kEmpty) {
+                                    vals[h] = 0.0;
+                                    break;
+                                }
+                                if (old == j) break;
+                                h = (h + 1) & (T - 1);
+                            }
+                        }
+                        wave_sync();
+                        while (ballot(pend)) {
+                            if (pend) atomicMin(&own[h], (uint32_t)l);
+                            wave_sync();
+                            if (pend && own[h] == (uint32_t)l) {
+                                vals[h] = __fma_rn(at, b, vals[h]);
+                                own[h] = kWave;
+                                pend = false;
+                            }
+                            wave_sync();
+                        }
+                    }
+                    continue;
+                }
+            }
             auto ins = [&](int32_t j, double b, double at) {
                 uint32_t h = hash_slot(j, kLog);
                 for (int probe = 0; probe < T; ++probe) {
@@ -708,10 +788,11 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
 // part of B_k inside the window (register cursor for rows with <= NT
 // entries, the item's cursor array otherwise), a workgroup scan flattens the
 // products in A-order, and each step applies NT of them.  Slot keys carry the
-// column offset in the high half and an ordering tag in the low half: two
-// products of one step that meet in a slot are applied in thread order (=
-// A-order) through atomicMin rounds.  A presence bitmap over the window gives
-// every entry its sorted rank (prefix popcounts) for the write-out.
+// column offset in the high bits and the slot's last pusher of the step in
+// the low bits: products of one step that meet in a slot form a list that
+// the last pusher applies in thread order (= A-order).  A presence bitmap
+// over the window gives every entry its sorted rank (prefix popcounts) for
+// the write-out.
 constexpr int kRegSlots = 4;  // register-held A entries per thread in the heavy numeric kernel
 constexpr int kSmallNT = 256;  // threads of the 2048-slot heavy numeric
 constexpr uint32_t kSlotFree = 0xffffffffu;
