@@ -132,7 +132,7 @@ int tr_spgemm_flops(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t 
  *  - light_max_flops: rows with more multiplications than this take the
  *    heavy (column-windowed) path, [64, 1024];
  *  - window_entries: heavy window packing quantum H (a window holds at most
- *    H + 127 entries), [64, heavy_table / 2 - 127];
+ *    H + 127 entries), [64, 3 * heavy_table / 4 - 127];
  *  - item_windows: heavy windows per work item, [1, 64];
  *  - heavy_table: slots of the heavy numeric table, 2048 (256-thread
  *    workgroups) or 4096 (512-thread workgroups).

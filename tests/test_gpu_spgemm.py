@@ -62,6 +62,7 @@ TUNINGS = [
     dict(heavy_table=4096, window_entries=64, item_windows=1),
     dict(window_entries=64, item_windows=64, light_max_flops=64),
     dict(window_entries=384, item_windows=4, light_max_flops=256),
+    dict(window_entries=1409, item_windows=2),
 ]
 
 

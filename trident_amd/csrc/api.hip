@@ -277,7 +277,7 @@ static void copy_tile(tr_ctx *ctx, const tr_csx_t *src, tr_csx_t *dst) {
 tr_tuning_t tuning_builtin() {
     tr_tuning_t t;
     t.light_max_flops = 1024;
-    t.window_entries = 896;
+    t.window_entries = 1408;
     t.item_windows = 16;
     t.heavy_table = 2048;
     return t;
@@ -297,8 +297,8 @@ void tuning_check(const tr_tuning_t &t) {
     TR_REQUIRE(t.light_max_flops >= 64 && t.light_max_flops <= 1024,
                "tuning: light_max_flops must be in [64, 1024]");
     TR_REQUIRE(t.heavy_table == 2048 || t.heavy_table == 4096, "tuning: heavy_table must be 2048 or 4096");
-    TR_REQUIRE(t.window_entries >= 64 && t.window_entries <= t.heavy_table / 2 - 127,
-               "tuning: window_entries must be in [64, heavy_table / 2 - 127]");
+    TR_REQUIRE(t.window_entries >= 64 && t.window_entries <= 3 * t.heavy_table / 4 - 127,
+               "tuning: window_entries must be in [64, 3 * heavy_table / 4 - 127]");
     TR_REQUIRE(t.item_windows >= 1 && t.item_windows <= 64, "tuning: item_windows must be in [1, 64]");
 }
 

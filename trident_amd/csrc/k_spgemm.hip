@@ -693,14 +693,23 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
                             const int32_t j = bi[q];
                             b = bv[q];
                             h = hash_slot(j, kLog);
+                            bool placed = false;
                             for (int probe = 0; probe < T; ++probe) {
                                 const int32_t old = atomicCAS(&keys[h], kEmpty, j);
                                 if (old == kEmpty) {
                                     vals[h] = 0.0;
+                                    placed = true;
                                     break;
                                 }
-                                if (old == j) break;
+                                if (old == j) {
+                                    placed = true;
+                                    break;
+                                }
                                 h = (h + 1) & (T - 1);
+                            }
+                            if (!placed) {
+                                atomicOr(&cnt->overflow, 2);  // table full: symbolic counts were wrong
+                                pend = false;
                             }
                         }
                         wave_sync();
@@ -1088,18 +1097,19 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             TR_GLOBAL double *ov = ov_row + W.off;
             const int nwords = dense ? (W.c1 - W.c0 + 63) >> 6 : kBmWords;
             // Each thread's entries as rank << 20 | column offset (kNone: no
-            // entry) and value.  The table's lower half [0, T / 2) is also the
-            // staging area (a window holds at most T / 2 entries): its slots
-            // are stashed in registers before a barrier, the upper half is
-            // read and staged after it.
+            // entry) and value.  The table's first three quarters [0, kStage)
+            // are also the staging area (a window holds at most kStage
+            // entries): their slots are stashed in registers before a
+            // barrier, the last quarter is read and staged after it.
             constexpr int kSt = (T + NT - 1) / NT;
-            constexpr int kLo = (T / 2 + NT - 1) / NT;
+            constexpr int kStage = 3 * T / 4;
+            constexpr int kLo = (kStage + NT - 1) / NT;
             constexpr int kSp = (kSparseH + 127 + NT - 1) / NT;  // sparse entries per thread
             constexpr int kStash = kLo > kSp ? kLo : kSp;
             constexpr uint32_t kNone = 0xffffffffu;
             uint32_t st_rk[kStash];
             double st_v[kStash];
-            const int32_t lim = min(W.cnt, T / 2);  // never write past the window
+            const int32_t lim = min(W.cnt, kStage);  // never write past the window
             int32_t *sidx = reinterpret_cast<int32_t *>(L.keytag);
             if (dense) {
                 // prefix popcounts of the presence words
@@ -1126,7 +1136,7 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                 for (int u = 0; u < kStash; ++u) {
                     const int h = tid + u * NT;
                     st_rk[u] = kNone;
-                    if (u >= kLo || h >= T / 2) continue;
+                    if (u >= kLo || h >= kStage) continue;
                     const uint32_t kt = L.keytag[h];
                     if (kt == kSlotFree) continue;
                     const uint32_t off = kt >> kTagBits;
@@ -1139,7 +1149,7 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
 #pragma unroll
                 for (int u = 0; u < kSt; ++u) {
                     const int h = tid + u * NT;
-                    if (h < T / 2 || h >= T) continue;
+                    if (h < kStage || h >= T) continue;
                     const uint32_t kt = L.keytag[h];
                     if (kt == kSlotFree) continue;
                     const uint32_t off = kt >> kTagBits;
