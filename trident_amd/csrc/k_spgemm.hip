@@ -372,7 +372,9 @@ __global__ __launch_bounds__(NTH) void k_sym_heavy(
             const int32_t width = (int32_t)min<int64_t>(1 << PB, (int64_t)ncols - base);
             const int32_t nwords = (width + 31) >> 5;
             const int32_t nfine = (width + (1 << kFineShift) - 1) >> kFineShift;
-            for (int t = tid; t < kWords; t += NTH) bm[t] = 0u;
+            // only the pass's words are marked and counted (a narrow matrix
+            // leaves the rest of the bitmap unused)
+            for (int t = tid; t < nwords; t += NTH) bm[t] = 0u;
             __syncthreads();
             // mark: entry pb + w + 16 t goes to wave w
             for (int32_t pb = a0; pb < a1; pb += kWavesB * kWave) {
