@@ -49,14 +49,17 @@ __device__ __forceinline__ RowSpan row_span(const AddTaskDev &T, int32_t g) {
     const int32_t r = g - T.row_base;
     RowSpan s;
     if (T.xp64) {
-        s.xs = T.xp64[g];
-        s.lx = (int32_t)(T.xp64[g + 1] - s.xs);
+        const TR_GLOBAL int64_t *xp64 = as_global(T.xp64);
+        s.xs = xp64[g];
+        s.lx = (int32_t)(xp64[g + 1] - s.xs);
     } else {
-        s.xs = T.xp[r];
-        s.lx = T.xp[r + 1] - T.xp[r];
+        const TR_GLOBAL int32_t *xp = as_global(T.xp);
+        s.xs = xp[r];
+        s.lx = xp[r + 1] - xp[r];
     }
-    s.ys = T.yp[r];
-    s.ly = T.yp[r + 1] - T.yp[r];
+    const TR_GLOBAL int32_t *yp = as_global(T.yp);
+    s.ys = yp[r];
+    s.ly = yp[r + 1] - yp[r];
     return s;
 }
 
@@ -76,8 +79,8 @@ __global__ void k_add_chunk_rows(const int64_t *__restrict__ cstart, int32_t row
 
 // merge-path split of diagonal t: number of X entries among the first t
 // merged positions (ties take X first)
-__device__ __forceinline__ int32_t merge_split(const int32_t *__restrict__ xi, int32_t lx,
-                                               const int32_t *__restrict__ yi, int32_t ly, int32_t t) {
+__device__ __forceinline__ int32_t merge_split(const TR_GLOBAL int32_t *__restrict__ xi, int32_t lx,
+                                               const TR_GLOBAL int32_t *__restrict__ yi, int32_t ly, int32_t t) {
     int32_t lo = t - ly > 0 ? t - ly : 0;
     int32_t hi = t < lx ? t : lx;
     while (lo < hi) {
@@ -123,10 +126,10 @@ __global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__
         const int32_t g = crow[c];
         const AddTaskDev &T = tasks[add_task_of(tasks, nt, g)];
         const RowSpan s = row_span(T, g);
-        const int32_t *__restrict__ xi = T.xi + s.xs;
-        const double *__restrict__ xv = T.xv + s.xs;
-        const int32_t *__restrict__ yi = T.yi + s.ys;
-        const double *__restrict__ yv = T.yv + s.ys;
+        const TR_GLOBAL int32_t *__restrict__ xi = as_global(T.xi) + s.xs;
+        const TR_GLOBAL double *__restrict__ xv = as_global(T.xv) + s.xs;
+        const TR_GLOBAL int32_t *__restrict__ yi = as_global(T.yi) + s.ys;
+        const TR_GLOBAL double *__restrict__ yv = as_global(T.yv) + s.ys;
         const int32_t t0 = (int32_t)(c - cstart[g]) * kChunk;
         const int32_t t1 = min(t0 + kChunk, s.lx + s.ly);
         int32_t sp = 0;
@@ -244,10 +247,11 @@ __global__ void k_add_rowptr(const AddTaskDev *__restrict__ tasks, int nt, int32
         const AddTaskDev &T = tasks[add_task_of(tasks, nt, g)];
         const int64_t base = coff[cstart[T.row_base]];
         const int32_t r = g - T.row_base;
-        T.optr[r] = (int32_t)(coff[cstart[g]] - base);
-        if (r == T.m - 1) T.optr[T.m] = (int32_t)(coff[cstart[g + 1]] - base);
+        TR_GLOBAL int32_t *optr = as_global(T.optr);
+        optr[r] = (int32_t)(coff[cstart[g]] - base);
+        if (r == T.m - 1) optr[T.m] = (int32_t)(coff[cstart[g + 1]] - base);
     }
-    if (g < nt && tasks[g].m == 0) tasks[g].optr[0] = 0;
+    if (g < nt && tasks[g].m == 0) as_global(tasks[g].optr)[0] = 0;
 }
 
 }  // namespace
