@@ -1208,9 +1208,11 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                 }
             }
             __syncthreads();
+            // streaming stores: C is written once and should not evict the B
+            // rows that later windows read again
             for (int e = tid; e < W.cnt; e += NT) {
-                oi[e] = sidx[e];
-                ov[e] = L.vals[e];
+                __builtin_nontemporal_store(sidx[e], &oi[e]);
+                __builtin_nontemporal_store(L.vals[e], &ov[e]);
             }
             __syncthreads();
             for (int t = tid; t < T; t += NT) L.keytag[t] = kSlotFree;
