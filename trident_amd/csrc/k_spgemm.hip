@@ -537,7 +537,8 @@ __global__ void k_item_cursor_len(const Item *__restrict__ items, const Counters
 // share its lines through L2 (the items of one row still run left to right
 // within their item).  Counting sort by column bucket: a block-aggregated
 // histogram, a scan, and a scatter with block-local ranks.
-constexpr int kOrderShift = 12;     // bucket = first window's c0 >> kOrderShift
+// bucket = first window's c0 >> shift; the host picks the smallest shift
+// that maps the widest operand onto the buckets
 constexpr int kOrderBuckets = 4096;
 
 __global__ void k_fill_u64(unsigned long long *__restrict__ p, int32_t n) {
@@ -545,20 +546,20 @@ __global__ void k_fill_u64(unsigned long long *__restrict__ p, int32_t n) {
 }
 
 __device__ __forceinline__ int item_bucket(const Item *__restrict__ items, const Window *__restrict__ wins,
-                                           int32_t it) {
+                                           int32_t it, int shift) {
     const int32_t c0 = wins[items[it].wbeg].c0;
-    return min(c0 >> kOrderShift, kOrderBuckets - 1);
+    return min(c0 >> shift, kOrderBuckets - 1);
 }
 
 __global__ __launch_bounds__(256) void k_item_hist(const Item *__restrict__ items, const Window *__restrict__ wins,
-                                                   const Counters *__restrict__ cnt,
+                                                   const Counters *__restrict__ cnt, int shift,
                                                    unsigned long long *__restrict__ hist) {
     __shared__ int32_t h[kOrderBuckets];
     for (int t = threadIdx.x; t < kOrderBuckets; t += 256) h[t] = 0;
     __syncthreads();
     const int32_t n = cnt->items;
     for (int32_t it = blockIdx.x * 256 + threadIdx.x; it < n; it += gridDim.x * 256)
-        atomicAdd(&h[item_bucket(items, wins, it)], 1);
+        atomicAdd(&h[item_bucket(items, wins, it, shift)], 1);
     __syncthreads();
     for (int t = threadIdx.x; t < kOrderBuckets; t += 256)
         if (h[t]) atomicAdd(&hist[t], (unsigned long long)h[t]);
@@ -591,7 +592,7 @@ __global__ __launch_bounds__(1024) void k_item_hist_scan(unsigned long long *__r
 // The same grid walk as k_item_hist: block-local ranks per bucket, one
 // global cursor bump per (block, bucket)
 __global__ __launch_bounds__(256) void k_item_scatter(const Item *__restrict__ items, const Window *__restrict__ wins,
-                                                      const Counters *__restrict__ cnt,
+                                                      const Counters *__restrict__ cnt, int shift,
                                                       unsigned long long *__restrict__ cursor,
                                                       int32_t *__restrict__ order) {
     __shared__ int32_t h[kOrderBuckets];
@@ -603,7 +604,7 @@ __global__ __launch_bounds__(256) void k_item_scatter(const Item *__restrict__ i
         const int32_t it = it0 + threadIdx.x;
         int b = 0, r = 0;
         if (it < n) {
-            b = item_bucket(items, wins, it);
+            b = item_bucket(items, wins, it, shift);
             r = atomicAdd(&h[b], 1);
         }
         __syncthreads();
@@ -1366,11 +1367,16 @@ int64_t mul_cursor_prep(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool capture) {
         pl.order_hist = (unsigned long long *)ws.get(S_ORDER_HIST, sizeof(unsigned long long) * kOrderBuckets);
     }
     const uint32_t og = cap_grid(div_up((uint64_t)pl.item_cap, 256), (int64_t)ctx->cu_count * 8);
+    int32_t maxn = 1;
+    for (const TaskDev &t : pl.h_tasks) maxn = std::max(maxn, t.n);
+    int shift = 0;
+    while (((int64_t)kOrderBuckets << shift) < (int64_t)maxn) ++shift;
     TR_LAUNCH(ctx, "item_order", k_fill_u64, div_up((uint64_t)kOrderBuckets, 256), 256, 0, pl.order_hist,
               kOrderBuckets);
-    TR_LAUNCH(ctx, "item_order", k_item_hist, og, 256, 0, pl.items, pl.wins, pl.cnt, pl.order_hist);
+    TR_LAUNCH(ctx, "item_order", k_item_hist, og, 256, 0, pl.items, pl.wins, pl.cnt, shift, pl.order_hist);
     TR_LAUNCH(ctx, "item_order", k_item_hist_scan, 1, 1024, 0, pl.order_hist);
-    TR_LAUNCH(ctx, "item_order", k_item_scatter, og, 256, 0, pl.items, pl.wins, pl.cnt, pl.order_hist, pl.order);
+    TR_LAUNCH(ctx, "item_order", k_item_scatter, og, 256, 0, pl.items, pl.wins, pl.cnt, shift, pl.order_hist,
+              pl.order);
     launch_exclusive_scan_i32_to_i64(ctx, pl.clen, pl.cbase, pl.item_cap, pl.scan_tmp);
     if (capture) return pl.cursor_cap;
     return read_scalar<int64_t>(ctx, pl.cbase + pl.item_cap);
