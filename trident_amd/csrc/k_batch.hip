@@ -237,7 +237,7 @@ void batch_body(tr_batch *b, bool capture) {
         if (b->mp.h.overflow & 1) fail(TR_ERR_HIP, "batch: heavy work list capacity exceeded");
         b->mp.cursor_cap = ncur;
         if (b->mp.h.items > 0)
-            b->mp.cursors = (int2 *)b->ws_mul->get(S_CURSORS, sizeof(int2) * (size_t)std::max<int64_t>(ncur, 1));
+            b->mp.cursors = (int4 *)b->ws_mul->get(S_CURSORS, sizeof(int4) * (size_t)std::max<int64_t>(ncur, 1));
         b->p_idx = (int32_t *)b->own(sizeof(int32_t) * (size_t)std::max<int64_t>(b->p_total, 1));
         b->p_val = (double *)b->own(sizeof(double) * (size_t)std::max<int64_t>(b->p_total, 1));
         Panel hp{};

@@ -797,15 +797,21 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
 // One NT-thread workgroup per item (a run of windows of one row),
 // sharing one T-slot LDS table per window.  For each window the row's A
 // entries are taken NT at a time (one per thread): each thread finds the
-// part of B_k inside the window (register cursor for rows with <= NT
+// part of B_k inside the window (register cursor for rows with <= 2 NT
 // entries, the item's cursor array otherwise), a workgroup scan flattens the
 // products in A-order, and each step applies NT of them.  Slot keys carry the
 // column offset in the high bits and the slot's last pusher of the step in
 // the low bits: products of one step that meet in a slot form a list that
 // the last pusher applies in thread order (= A-order).  A presence bitmap
-// over the window gives every entry its sorted rank (prefix popcounts) for
-// the write-out.
-constexpr int kRegSlots = 4;  // register-held A entries per thread in the heavy numeric kernel
+// over the window, built after the window's last step, gives every entry its
+// sorted rank (prefix popcounts) for the write-out.
+//
+// Occupancy: the kernel is bound by the latency of its dependent LDS and
+// memory chains, so resident waves matter most.  The bitmap shares LDS with
+// the step arrays (31 KB per 2048-slot, 256-thread workgroup: five per CU)
+// and the register budget is held to 96 VGPRs (five waves per SIMD); the
+// 4096-slot table (62 KB, 512 threads) fits two workgroups per CU.
+constexpr int kRegSlots = 2;  // register-held A entries per thread in the heavy numeric kernel
 constexpr int kSmallNT = 256;  // threads of the 2048-slot heavy numeric
 constexpr uint32_t kSlotFree = 0xffffffffu;
 constexpr int kTagBits = 10;                        // thread ids < 1023
@@ -815,19 +821,25 @@ template <int T, int NT>
 struct HeavyWgLds {
     uint32_t keytag[T];
     double vals[T];
-    unsigned long long bm[kBmWords];
-    uint16_t wpre[kBmWords];
-    int32_t seg_d[NT];  // B position minus flattened product position of segment (A entry) o
     int32_t wsum[NT / kWave];
     int32_t nocc;
-    double pa[NT], pb[NT];  // the step's products (A value, B value) by thread
-    uint16_t link[NT];      // next older product of the same slot in this step
-    // segment starts of three rotating product steps: bit p of smask[b] is
-    // set where a segment starts at step position p (position 0 always holds
-    // the segment that covers the step's first product), smark[b][p] is
-    // that segment's index
-    unsigned long long smask[3][NT / kWave];
-    typename std::conditional<(NT <= 256), uint8_t, uint16_t>::type smark[3][NT];
+    union {
+        struct {                    // product steps of a window
+            double pa[NT], pb[NT];  // the step's products (A value, B value) by thread
+            // segment starts of three rotating product steps: bit p of
+            // smask[b] is set where a segment starts at step position p
+            // (position 0 always holds the segment that covers the step's
+            // first product), smark[b][p] is that segment's index
+            unsigned long long smask[3][NT / kWave];
+            int32_t seg_d[NT];  // B position minus flattened product position of segment (A entry) o
+            uint16_t link[NT];  // next older product of the same slot in this step
+            typename std::conditional<(NT <= 256), uint8_t, uint16_t>::type smark[3][NT];
+        };
+        struct {  // extraction of a window: presence bitmap and its prefix counts
+            unsigned long long bm[kBmWords];
+            uint16_t wpre[kBmWords];
+        };
+    };
 };
 
 // marks the segment [excl, excl + len) of the calling thread in the step that
@@ -874,18 +886,17 @@ __device__ __forceinline__ int32_t wg_excl_scan(int32_t v, int32_t *wsum, int32_
 }
 
 template <int T, int NT>
-__global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__ tasks, int nt,
+__global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const TaskDev *__restrict__ tasks, int nt,
                                                      const Item *__restrict__ items, Counters *__restrict__ cnt,
                                                      const Window *__restrict__ wins,
                                                      const int32_t *__restrict__ order,
                                                      const int64_t *__restrict__ cursor_base,
-                                                     int2 *__restrict__ cursors, const int64_t *__restrict__ rowptr,
+                                                     int4 *__restrict__ cursors, const int64_t *__restrict__ rowptr,
                                                      const Panel *__restrict__ panels, int npanels) {
     constexpr uint32_t kLog = __builtin_ctz(T);
     __shared__ HeavyWgLds<T, NT> L;
     const int tid = threadIdx.x, l = lane_id();
     for (int t = tid; t < T; t += NT) L.keytag[t] = kSlotFree;
-    for (int t = tid; t < kBmWords; t += NT) L.bm[t] = 0ull;
     if (tid == 0) L.nocc = 0;
     __syncthreads();
     const int32_t nitems = cnt->items;
@@ -903,7 +914,9 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
         const TR_GLOBAL double *__restrict__ bv = as_global(TK.bv);
         const int32_t ncols = TK.n;
         const int32_t a0 = as_global(TK.ap)[r], a1 = as_global(TK.ap)[r + 1];
-        int2 *cur = cursors + cursor_base[itx];
+        // cursor of A entry p: (first B position at or after the next window,
+        // its column, end of B_k)
+        int4 *cur = cursors + cursor_base[itx];
         const int64_t rs = rowptr[g];
         const Panel &P = find_panel(panels, npanels, g);
         TR_GLOBAL int32_t *oi_row = as_global(P.idx) + (rs - P.nnz_begin);
@@ -955,15 +968,15 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                                 v = rk_nxt[u];
                             }
                         bs = c;
+                    } else if (have_cursor && !full) {
+                        const int4 cv = cur[p - a0];
+                        c = cv.x;
+                        v = cv.y;
+                        be = cv.z;
                     } else {
                         const int32_t k = ai[p];
                         bs = bp[k];
                         be = bp[k + 1];
-                        if (have_cursor && !full) {
-                            const int2 cv = cur[p - a0];
-                            c = cv.x;
-                            v = cv.y;
-                        }
                     }
                     if (full) {
                         s = bs;
@@ -984,7 +997,7 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                                     rk_nxt[u] = v;
                                 }
                         } else if (!last) {
-                            cur[p - a0] = make_int2(e, v);
+                            cur[p - a0] = make_int4(e, v, be, 0);
                         }
                     }
                 }
@@ -1045,7 +1058,6 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
                         uint32_t prev = kNoTag;
                         if (old == kSlotFree) {
                             L.vals[h] = 0.0;
-                            if (dense) atomicOr(&L.bm[off >> 6], 1ull << (off & 63));
                             ++nfresh;
                         } else if ((old >> kTagBits) == off) {
                             prev = atomicExch(&L.keytag[h], me) & kNoTag;
@@ -1115,6 +1127,20 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             const int32_t lim = min(W.cnt, kStage);  // never write past the window
             int32_t *sidx = reinterpret_cast<int32_t *>(L.keytag);
             if (dense) {
+                // presence bitmap of the window's columns (it shares LDS with
+                // the step arrays, which are dead now)
+                for (int t = tid; t < nwords; t += NT) L.bm[t] = 0ull;
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < kSt; ++u) {
+                    const int h = tid + u * NT;
+                    if (h >= T) continue;
+                    const uint32_t kt = L.keytag[h];
+                    if (kt == kSlotFree) continue;
+                    const uint32_t off = kt >> kTagBits;
+                    atomicOr(&L.bm[off >> 6], 1ull << (off & 63));
+                }
+                __syncthreads();
                 // prefix popcounts of the presence words
                 constexpr int kPer = (kBmWords + NT - 1) / NT;
                 int32_t wc[kPer];
@@ -1217,7 +1243,6 @@ __global__ __launch_bounds__(NT) void k_num_heavy_wg(const TaskDev *__restrict__
             }
             __syncthreads();
             for (int t = tid; t < T; t += NT) L.keytag[t] = kSlotFree;
-            for (int t = tid; t < nwords; t += NT) L.bm[t] = 0ull;  // also the key list of a sparse window
             if (tid == 0) L.nocc = 0;
             __syncthreads();
         }
@@ -1539,7 +1564,7 @@ int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max
     Panel *dp = (Panel *)ws.get(S_TMP_CSX, sizeof(Panel) * (size_t)np);
     TR_HIP(hipMemcpyAsync(dp, hp.data(), sizeof(Panel) * (size_t)np, hipMemcpyHostToDevice, ctx->stream));
     if (pl.h.items > 0)
-        pl.cursors = (int2 *)ws.get(S_CURSORS, sizeof(int2) * (size_t)std::max<int64_t>(ncur, 1));
+        pl.cursors = (int4 *)ws.get(S_CURSORS, sizeof(int4) * (size_t)std::max<int64_t>(ncur, 1));
     mul_numeric(ctx, pl, dp, np);
     mul_check(ctx, pl);
     return TR_OK;

@@ -104,7 +104,7 @@ struct MulPlan {
     int32_t win_cap = 0, item_cap = 0;
     int32_t *clen = nullptr;
     int64_t *cbase = nullptr;
-    int2 *cursors = nullptr;  // (cursor, column at the cursor) per A entry of multi-window items
+    int4 *cursors = nullptr;  // (cursor, column at the cursor, end of B_k) per A entry of multi-window items
     int32_t *order = nullptr;          // heavy items sorted by the column of their first window
     unsigned long long *order_hist = nullptr;
     int64_t cursor_cap = 0;
