@@ -162,6 +162,33 @@ def test_heavy_sparse_rows(ctx, ncols):
     run(ctx, A, B, f"sparse heavy {ncols}")
 
 
+def test_heavy_window_bitmap_pass_edges(ctx):
+    # windows up to 2^19 columns wide are ranked one 32768-column presence
+    # bitmap pass at a time: entries on both sides of every pass edge, at
+    # the window-width quantum and at the matrix's last column must keep
+    # their sorted places.  Rows of 40..400 A entries put both row classes
+    # (2^19- and 2^20-column symbolic passes) over windows of 3 to 16 passes.
+    rng = np.random.default_rng(23)
+    ncols = (1 << 20) + 300
+    edges = np.arange(32768, ncols, 32768)
+    special = np.unique(np.concatenate([edges - 1, edges, [0, ncols - 1]]))
+    k = 500
+    bl = []
+    for _ in range(k):
+        extra = rng.choice(ncols, size=int(rng.integers(10, 60)), replace=False)
+        pick = special[rng.random(special.size) < 0.3]
+        bl.append(np.unique(np.concatenate([extra, pick])))
+    bptr = np.concatenate([[0], np.cumsum([len(x) for x in bl])])
+    bval = rng.uniform(-1, 1, bptr[-1]) * np.exp2(rng.integers(-20, 21, bptr[-1]))
+    B = gen.from_arrays(k, ncols, bptr, np.concatenate(bl), bval)
+    al = [np.sort(rng.choice(k, size=int(rng.integers(40, 200 if i % 2 else 400)), replace=False))
+          for i in range(48)]
+    aptr = np.concatenate([[0], np.cumsum([len(x) for x in al])])
+    A = gen.from_arrays(48, k, aptr, np.concatenate(al), rng.uniform(-1, 1, aptr[-1]))
+    C = run(ctx, A, B, "bitmap pass edges")
+    assert np.isin(special, C.idx).all()
+
+
 def test_galerkin_operators(ctx):
     # the AMG products A*P and P^T*(A*P): light rows whose B rows hold about
     # 4.5 entries, multiplied with their products flattened over the lanes

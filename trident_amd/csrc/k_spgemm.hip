@@ -943,9 +943,10 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
             if (W.cnt == 0) continue;  // block-uniform
             const bool full = (W.c0 == 0 && W.c1 >= ncols);
             const bool last = (wi == I.wend - 1);
-            // windows up to kWinMaxWidth columns are ranked by the presence
-            // bitmap; wider (sparse) windows hold <= kSparseH + 127 entries
-            // and are ranked by counting smaller keys
+            // windows up to kWinMaxWidth columns are ranked by presence
+            // bitmaps of kBmWidth columns, one pass each; wider (sparse)
+            // windows hold <= kSparseH + 127 entries and are ranked by
+            // counting smaller keys
             const bool dense = (W.c1 - W.c0) <= kWinMaxWidth;
             int nfresh = 0;
             for (int32_t pb = a0, slot = 0; pb < a1; pb += NT, ++slot) {
@@ -1110,83 +1111,91 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
             if (l == 0 && wf) atomicAdd(&L.nocc, wf);
             TR_GLOBAL int32_t *oi = oi_row + W.off;
             TR_GLOBAL double *ov = ov_row + W.off;
-            const int nwords = dense ? (W.c1 - W.c0 + 63) >> 6 : kBmWords;
             // Each thread's entries as rank << 20 | column offset (kNone: no
             // entry) and value.  The table's first three quarters [0, kStage)
             // are also the staging area (a window holds at most kStage
-            // entries): their slots are stashed in registers before a
+            // entries): their values are stashed in registers before a
             // barrier, the last quarter is read and staged after it.
             constexpr int kSt = (T + NT - 1) / NT;
             constexpr int kStage = 3 * T / 4;
             constexpr int kLo = (kStage + NT - 1) / NT;
             constexpr int kSp = (kSparseH + 127 + NT - 1) / NT;  // sparse entries per thread
             constexpr int kStash = kLo > kSp ? kLo : kSp;
+            static_assert(kSp <= kSt && kStage % NT == 0, "stash slots are exactly the staging area's");
             constexpr uint32_t kNone = 0xffffffffu;
-            uint32_t st_rk[kStash];
+            uint32_t st_rk[kSt];
             double st_v[kStash];
             const int32_t lim = min(W.cnt, kStage);  // never write past the window
             int32_t *sidx = reinterpret_cast<int32_t *>(L.keytag);
             if (dense) {
-                // presence bitmap of the window's columns (it shares LDS with
-                // the step arrays, which are dead now)
-                for (int t = tid; t < nwords; t += NT) L.bm[t] = 0ull;
-                __syncthreads();
+                // the thread's slots (h = tid + u NT) by column offset, ranked
+                // one kBmWidth-column pass at a time with a presence bitmap
+                // (it shares LDS with the step arrays, which are dead now)
 #pragma unroll
                 for (int u = 0; u < kSt; ++u) {
                     const int h = tid + u * NT;
-                    if (h >= T) continue;
-                    const uint32_t kt = L.keytag[h];
-                    if (kt == kSlotFree) continue;
-                    const uint32_t off = kt >> kTagBits;
-                    atomicOr(&L.bm[off >> 6], 1ull << (off & 63));
+                    const uint32_t kt = h < T ? L.keytag[h] : kSlotFree;
+                    st_rk[u] = kt == kSlotFree ? kNone : kt >> kTagBits;
                 }
-                __syncthreads();
-                // prefix popcounts of the presence words
-                constexpr int kPer = (kBmWords + NT - 1) / NT;
-                int32_t wc[kPer];
-                int32_t tsum = 0;
+                const int32_t width = W.c1 - W.c0;
+                int32_t base_rank = 0;
+                for (int32_t lo = 0; lo < width; lo += kBmWidth) {  // block-uniform
+                    const int nwords = min(kBmWords, (width - lo + 63) >> 6);
+                    for (int t = tid; t < nwords; t += NT) L.bm[t] = 0ull;
+                    __syncthreads();
 #pragma unroll
-                for (int u = 0; u < kPer; ++u) {
-                    const int wd = tid * kPer + u;
-                    wc[u] = (wd < nwords && wd < kBmWords) ? __popcll(L.bm[wd]) : 0;
-                    tsum += wc[u];
-                }
-                int32_t tot;
-                int32_t run = wg_excl_scan<NT>(tsum, L.wsum, &tot);
+                    for (int u = 0; u < kSt; ++u) {
+                        const uint32_t rel = st_rk[u] - (uint32_t)lo;  // kNone wraps past kBmWidth
+                        if (rel < (uint32_t)kBmWidth) atomicOr(&L.bm[rel >> 6], 1ull << (rel & 63));
+                    }
+                    __syncthreads();
+                    // prefix popcounts of the presence words
+                    constexpr int kPer = (kBmWords + NT - 1) / NT;
+                    int32_t wc[kPer];
+                    int32_t tsum = 0;
 #pragma unroll
-                for (int u = 0; u < kPer; ++u) {
-                    const int wd = tid * kPer + u;
-                    if (wd < kBmWords) L.wpre[wd] = (uint16_t)run;
-                    run += wc[u];
+                    for (int u = 0; u < kPer; ++u) {
+                        const int wd = tid * kPer + u;
+                        wc[u] = (wd < nwords) ? __popcll(L.bm[wd]) : 0;
+                        tsum += wc[u];
+                    }
+                    int32_t tot;
+                    int32_t run = wg_excl_scan<NT>(tsum, L.wsum, &tot);
+#pragma unroll
+                    for (int u = 0; u < kPer; ++u) {
+                        const int wd = tid * kPer + u;
+                        if (wd < kBmWords) L.wpre[wd] = (uint16_t)run;
+                        run += wc[u];
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int u = 0; u < kSt; ++u) {
+                        const uint32_t rel = st_rk[u] - (uint32_t)lo;
+                        if (rel < (uint32_t)kBmWidth) {
+                            const int wd = (int)(rel >> 6);
+                            const uint32_t rr = (uint32_t)(base_rank + L.wpre[wd]) +
+                                                __popcll(L.bm[wd] & ((1ull << (rel & 63)) - 1ull));
+                            st_rk[u] |= rr << 20;
+                        }
+                    }
+                    base_rank += tot;
+                    __syncthreads();  // wsum, bm and wpre are reused by the next pass
                 }
-                __syncthreads();
                 if (tid == 0 && L.nocc != W.cnt) atomicOr(&cnt->overflow, 4);
 #pragma unroll
                 for (int u = 0; u < kStash; ++u) {
                     const int h = tid + u * NT;
-                    st_rk[u] = kNone;
-                    if (u >= kLo || h >= kStage) continue;
-                    const uint32_t kt = L.keytag[h];
-                    if (kt == kSlotFree) continue;
-                    const uint32_t off = kt >> kTagBits;
-                    const int wd = (int)(off >> 6);
-                    const uint32_t rr = (uint32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
-                    st_rk[u] = (rr << 20) | off;
-                    st_v[u] = L.vals[h];
+                    if (u < kLo && h < kStage && st_rk[u] != kNone) st_v[u] = L.vals[h];
                 }
                 __syncthreads();
 #pragma unroll
                 for (int u = 0; u < kSt; ++u) {
                     const int h = tid + u * NT;
-                    if (h < kStage || h >= T) continue;
-                    const uint32_t kt = L.keytag[h];
-                    if (kt == kSlotFree) continue;
-                    const uint32_t off = kt >> kTagBits;
-                    const int wd = (int)(off >> 6);
-                    const int32_t rr = (int32_t)L.wpre[wd] + __popcll(L.bm[wd] & ((1ull << (off & 63)) - 1ull));
+                    if (h < kStage || h >= T || st_rk[u] == kNone) continue;
+                    const int32_t rr = (int32_t)(st_rk[u] >> 20);
                     if (rr < lim) {
                         const double v = L.vals[h];
-                        sidx[rr] = W.c0 + (int32_t)off;
+                        sidx[rr] = W.c0 + (int32_t)(st_rk[u] & 0xfffffu);
                         L.vals[rr] = v;
                     }
                 }

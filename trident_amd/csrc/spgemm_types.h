@@ -21,9 +21,12 @@ constexpr int kMidPassBits = 19;            // mid rows: 2^19-column passes, two
 constexpr int kMidThreads = 512;
 constexpr int kPassBits = 20;      // heavy bitmap covers 2^20 columns per pass
 constexpr int kFineShift = 7;      // fine block = 128 columns
-constexpr int kWinMaxWidth = 32768;                       // window width cap (columns)
-constexpr int kWidthShift = 15 - kFineShift;               // fine blocks per width quantum: 2^8
-constexpr int kBmWords = kWinMaxWidth / 64;                // presence words per window
+constexpr int kWinLog = 19;                                // window width cap: 2^19 columns
+constexpr int kWinMaxWidth = 1 << kWinLog;                 // (column offsets stay below 2^20)
+constexpr int kWidthShift = kWinLog - kFineShift;          // fine blocks per width quantum
+constexpr int kBmWidth = 32768;                            // columns ranked per presence-bitmap pass
+constexpr int kBmWords = kBmWidth / 64;                    // presence words per pass
+static_assert(kWinMaxWidth <= (1 << 20) && kPassBits <= 20, "window column offsets fit 20 bits");
 constexpr int kSparseAvg = 192;    // a pass with <= this many entries per width quantum is sparse
 constexpr int kSparseH = 256;      // sparse windows: count quantum (<= kSparseH + 127 entries, any width)
 constexpr int kHeavyThreads = 1024;  // symbolic heavy block
