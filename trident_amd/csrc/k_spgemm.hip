@@ -797,7 +797,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
 // One NT-thread workgroup per item (a run of windows of one row),
 // sharing one T-slot LDS table per window.  For each window the row's A
 // entries are taken NT at a time (one per thread): each thread finds the
-// part of B_k inside the window (register cursor for rows with <= 2 NT
+// part of B_k inside the window (register cursor for rows with <= NT
 // entries, the item's cursor array otherwise), a workgroup scan flattens the
 // products in A-order, and each step applies NT of them.  Slot keys carry the
 // column offset in the high bits and the slot's last pusher of the step in
@@ -811,7 +811,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
 // the step arrays (31 KB per 2048-slot, 256-thread workgroup: five per CU)
 // and the register budget is held to 96 VGPRs (five waves per SIMD); the
 // 4096-slot table (62 KB, 512 threads) fits two workgroups per CU.
-constexpr int kRegSlots = 2;  // register-held A entries per thread in the heavy numeric kernel
+constexpr int kRegSlots = 1;  // register-held A entries per thread in the heavy numeric kernel
 constexpr int kSmallNT = 256;  // threads of the 2048-slot heavy numeric
 constexpr uint32_t kSlotFree = 0xffffffffu;
 constexpr int kTagBits = 10;                        // thread ids < 1023
@@ -984,7 +984,9 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
                         e = be;
                     } else {
                         if (!have_cursor) {
-                            c = lower_bound_dev(bi, bs, be, W.c0);
+                            // an item that starts at the row's first column
+                            // starts at the front of B_k: no search
+                            c = W.c0 == 0 ? bs : lower_bound_dev(bi, bs, be, W.c0);
                             v = c < be ? bi[c] : INT32_MAX;
                         }
                         s = c;
