@@ -57,7 +57,7 @@ def test_tuning_defaults_and_validation_without_gpu():
     lib = _lib.load()
     t = _lib.tr_tuning_t()
     assert lib.tr_tuning_default(ctypes.byref(t)) == _lib.TR_OK
-    assert (t.light_max_flops, t.window_entries, t.item_windows, t.heavy_table) == (1024, 1408, 16, 2048)
+    assert (t.light_max_flops, t.window_entries, t.item_windows, t.heavy_table) == (1024, 1152, 16, 2048)
     bad = [dict(light_max_flops=2048), dict(light_max_flops=32), dict(heavy_table=3000),
            dict(window_entries=1410), dict(window_entries=32), dict(item_windows=0), dict(item_windows=65)]
     for b in bad:

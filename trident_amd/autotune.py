@@ -51,7 +51,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING_DIR = os.path.join(HERE, "tuning")
 
 # search space
-TABLE_WINDOWS = [(2048, 384), (2048, 640), (2048, 896), (2048, 1152), (2048, 1408), (4096, 1920), (4096, 2816)]
+TABLE_WINDOWS = [(2048, 384), (2048, 640), (2048, 896), (2048, 1024), (2048, 1152), (2048, 1280), (2048, 1408), (4096, 1920), (4096, 2816)]
 ITEM_WINDOWS = [1, 2, 4, 8, 16, 32]
 LIGHT_MAX = [256, 512, 1024]
 

@@ -277,7 +277,7 @@ static void copy_tile(tr_ctx *ctx, const tr_csx_t *src, tr_csx_t *dst) {
 tr_tuning_t tuning_builtin() {
     tr_tuning_t t;
     t.light_max_flops = 1024;
-    t.window_entries = 1408;
+    t.window_entries = 1152;
     t.item_windows = 16;
     t.heavy_table = 2048;
     return t;
