@@ -162,12 +162,24 @@ def test_heavy_sparse_rows(ctx, ncols):
     run(ctx, A, B, f"sparse heavy {ncols}")
 
 
-def test_heavy_window_bitmap_pass_edges(ctx):
+@pytest.mark.parametrize("tuning", [None, dict(heavy_table=4096, window_entries=2816)],
+                         ids=["table2048", "table4096"])
+def test_heavy_window_bitmap_pass_edges(ctx, tuning):
     # windows up to 2^19 columns wide are ranked one 32768-column presence
     # bitmap pass at a time: entries on both sides of every pass edge, at
     # the window-width quantum and at the matrix's last column must keep
     # their sorted places.  Rows of 40..400 A entries put both row classes
-    # (2^19- and 2^20-column symbolic passes) over windows of 3 to 16 passes.
+    # (2^19- and 2^20-column symbolic passes) over windows of 3 to 16 passes,
+    # with both heavy table sizes.
+    if tuning:
+        ctx.set_tuning(dict(ctx.get_tuning(), **tuning))
+    try:
+        _pass_edges(ctx)
+    finally:
+        ctx.set_tuning(None)
+
+
+def _pass_edges(ctx):
     rng = np.random.default_rng(23)
     ncols = (1 << 20) + 300
     edges = np.arange(32768, ncols, 32768)

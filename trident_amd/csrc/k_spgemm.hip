@@ -347,9 +347,9 @@ __global__ __launch_bounds__(NTH) void k_sym_heavy(
     int32_t win_cap, Item *__restrict__ items, int32_t item_cap) {
     constexpr int kWords = 1 << (PB - 5);          // bitmap words of one pass
     constexpr int kFine = 1 << (PB - kFineShift);  // fine blocks of one pass
-    constexpr int kPerThread = kFine / NTH;        // 8
+    constexpr int kPerThread = kFine / NTH;        // consecutive fine blocks per thread: 2..8
     constexpr int kWavesB = NTH / kWave;
-    static_assert(kPerThread == 8, "eight fine blocks per thread");
+    static_assert(kPerThread >= 1 && kPerThread <= 8, "fine blocks per thread");
     __shared__ uint32_t bm[kWords];
     __shared__ int32_t wsum[kWavesB];
     __shared__ int32_t sh_wbase;
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(NTH) void k_sym_heavy(
                 }
             }
             __syncthreads();
-            // fine-block counts, 8 consecutive fine blocks per thread
+            // fine-block counts, kPerThread consecutive fine blocks per thread
             int32_t c[kPerThread];
             int32_t tsum = 0;
 #pragma unroll
@@ -1349,6 +1349,13 @@ void mul_analyze(tr_ctx *ctx, MulPlan &pl) {
 
 void mul_read_counts(tr_ctx *ctx, MulPlan &pl) { read_array(ctx, &pl.h, pl.cnt, sizeof(Counters)); }
 
+template <int PB, int NTH>
+void launch_sym_heavy(tr_ctx *ctx, MulPlan &pl, int cls, int per_cu, bool emit) {
+    TR_LAUNCH(ctx, "sym_heavy", (k_sym_heavy<PB, NTH>), cap_grid(pl.h.cls[cls], (int64_t)ctx->cu_count * per_cu), NTH,
+              0, pl.d_tasks, pl.nt, pl.lists + (int64_t)cls * pl.rows_total, pl.cnt, cls, pl.rownnz, emit ? 1 : 0,
+              pl.tune.window_entries, pl.tune.item_windows, pl.wins, pl.win_cap, pl.items, pl.item_cap);
+}
+
 void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool capture) {
     if (pl.h.cls[0]) launch_sym_light<64>(ctx, pl, 0);
     if (pl.h.cls[1]) launch_sym_light<128>(ctx, pl, 1);
@@ -1358,9 +1365,9 @@ void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool captu
     if (pl.h.cls[5]) launch_sym_light<2048>(ctx, pl, 5);
     const int32_t nh = pl.h.cls[kHeavyClass] + pl.h.cls[kMidClass];
     if (nh) {
+        int32_t max_n = 0;
+        for (auto &t : pl.h_tasks) max_n = std::max(max_n, t.n);
         if (emit && !capture) {
-            int32_t max_n = 0;
-            for (auto &t : pl.h_tasks) max_n = std::max(max_n, t.n);
             // windows per pass <= ceil(pass_nnz / H) + width splits (mid rows use the shorter pass)
             const int64_t passes = ((int64_t)max_n + (1 << kMidPassBits) - 1) >> kMidPassBits;
             const int64_t wsplits = (int64_t)max_n / kWinMaxWidth + 1;
@@ -1373,17 +1380,24 @@ void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool captu
             pl.wins = (Window *)ws.get(S_WINDOWS, sizeof(Window) * (size_t)wcap);
             pl.items = (Item *)ws.get(S_ITEMS, sizeof(Item) * (size_t)icap);
         }
-        if (pl.h.cls[kHeavyClass])
-            TR_LAUNCH(ctx, "sym_heavy", (k_sym_heavy<kPassBits, kHeavyThreads>),
-                      cap_grid(pl.h.cls[kHeavyClass], (int64_t)ctx->cu_count * 2), kHeavyThreads, 0, pl.d_tasks,
-                      pl.nt, pl.lists + (int64_t)kHeavyClass * pl.rows_total, pl.cnt, kHeavyClass, pl.rownnz,
-                      emit ? 1 : 0, pl.tune.window_entries, pl.tune.item_windows, pl.wins, pl.win_cap, pl.items,
-                      pl.item_cap);
-        if (pl.h.cls[kMidClass])
-            TR_LAUNCH(ctx, "sym_heavy", (k_sym_heavy<kMidPassBits, kMidThreads>),
-                      cap_grid(pl.h.cls[kMidClass], (int64_t)ctx->cu_count * 4), kMidThreads, 0, pl.d_tasks, pl.nt,
-                      pl.lists + (int64_t)kMidClass * pl.rows_total, pl.cnt, kMidClass, pl.rownnz, emit ? 1 : 0,
-                      pl.tune.window_entries, pl.tune.item_windows, pl.wins, pl.win_cap, pl.items, pl.item_cap);
+        // The bitmap covers 2^PB columns per pass and its LDS bounds the
+        // resident workgroups: 2^20 bits (128 KiB) hold one per CU, 2^19 two,
+        // 2^18 four.  The narrowest pass that still covers the widest
+        // operand in one pass is used (same windows, same results).  The
+        // grids hold many more workgroups than fit at once: rows differ
+        // widely in cost, and queued workgroups go to whichever CU frees
+        // first (Galerkin 128^3: 12.9 -> 12.3 ms against one resident wave
+        // of workgroups).
+        const int pb = max_n <= (1 << 18) ? 18 : max_n <= (1 << 19) ? 19 : kPassBits;
+        if (pl.h.cls[kHeavyClass]) {
+            if (pb == 18) launch_sym_heavy<18, kMidThreads>(ctx, pl, kHeavyClass, 64, emit);
+            else if (pb == 19) launch_sym_heavy<19, kHeavyThreads>(ctx, pl, kHeavyClass, 32, emit);
+            else launch_sym_heavy<kPassBits, kHeavyThreads>(ctx, pl, kHeavyClass, 32, emit);
+        }
+        if (pl.h.cls[kMidClass]) {
+            if (pb == 18) launch_sym_heavy<18, kMidThreads>(ctx, pl, kMidClass, 64, emit);
+            else launch_sym_heavy<kMidPassBits, kMidThreads>(ctx, pl, kMidClass, 64, emit);
+        }
     }
     launch_exclusive_scan_i64(ctx, pl.rownnz, pl.rowptr, pl.rows_total, pl.scan_tmp);
 }
