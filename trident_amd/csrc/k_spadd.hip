@@ -118,12 +118,12 @@ __global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__
     constexpr int kVN = kWrite ? kSub + 2 : 1;
     __shared__ int32_t s_xi[4][kSub + 2], s_yi[4][kSub + 2];
     __shared__ double s_xv[kVW][kVN], s_yv[kVW][kVN];
-    const int l = lane_id(), w = threadIdx.x / kWave;
+    const int l = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
     int32_t *XI = s_xi[w], *YI = s_yi[w];
     double *XV = s_xv[kWrite ? w : 0], *YV = s_yv[kWrite ? w : 0];
     const int64_t nchunks = cstart[rows];
     for (int64_t c = (int64_t)blockIdx.x * 4 + w; c < nchunks; c += (int64_t)gridDim.x * 4) {
-        const int32_t g = crow[c];
+        const int32_t g = __builtin_amdgcn_readfirstlane(crow[c]);
         const AddTaskDev &T = tasks[add_task_of(tasks, nt, g)];
         const RowSpan s = row_span(T, g);
         const TR_GLOBAL int32_t *__restrict__ xi = as_global(T.xi) + s.xs;

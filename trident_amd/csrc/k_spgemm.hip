@@ -270,6 +270,8 @@ __global__ __launch_bounds__(256) void k_sym_light(const TaskDev *__restrict__ t
                                                    int cls, int64_t *__restrict__ rownnz) {
     __shared__ int32_t tabs[4][T];
     constexpr uint32_t kLog = __builtin_ctz(T);
+    // (marking the row wave-uniform here moves its values to scalar
+    // registers, 101 SGPRs: 7 waves per SIMD instead of 8, and slower)
     const int w = threadIdx.x / kWave, l = lane_id();
     int32_t *tab = tabs[w];
     const int32_t nrows = cnt->cls[cls];
@@ -634,22 +636,35 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
                                                            const int64_t *__restrict__ rowptr,
                                                            const Panel *__restrict__ panels, int npanels) {
     // tables of <= 512 slots also flatten chunks of short B rows over the
-    // lanes; their per-slot owner words order the chunk's products
+    // lanes; per-slot words order the chunk's products: lane masks (<= 256
+    // slots) or owner lanes (512 slots, where 64-bit masks would cost
+    // occupancy)
     constexpr bool kFlat = T <= 512;
+    constexpr bool kMask = T <= 256;
     constexpr int kOwn = kFlat ? T : 1;
+    using OwnT = typename std::conditional<kMask, unsigned long long, uint32_t>::type;
     __shared__ int32_t keys_s[WPB][T];
     __shared__ double vals_s[WPB][T];
-    __shared__ uint32_t own_s[WPB][kOwn];
+    __shared__ OwnT own_s[WPB][kOwn];
     constexpr uint32_t kLog = __builtin_ctz(T);
-    const int w = threadIdx.x / kWave, l = lane_id();
+    // the wave index and everything per row are wave-uniform (scalar loads
+    // and registers)
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), l = lane_id();
     int32_t *keys = keys_s[w];
     double *vals = vals_s[w];
-    uint32_t *own = own_s[w];
+    OwnT *own = own_s[w];
     if (kFlat)
-        for (int t = l; t < kOwn; t += kWave) own[t] = kWave;
+        for (int t = l; t < kOwn; t += kWave) own[t] = kMask ? 0 : kWave;
     const int32_t nrows = cnt->cls[cls];
-    for (int32_t li = blockIdx.x * WPB + w; li < nrows; li += gridDim.x * WPB) {
-        const int32_t g = rows[li];
+    const int32_t stride = gridDim.x * WPB;
+    int32_t g_next = 0;
+    if (blockIdx.x * WPB + w < nrows) g_next = rows[blockIdx.x * WPB + w];
+    for (int32_t li = blockIdx.x * WPB + w; li < nrows; li += stride) {
+        // a row is a chain of dependent loads (row, A entries, B row
+        // pointers, B entries); the next row's index and this row's output
+        // place are loaded early so that their latency overlaps the chain
+        const int32_t g = __builtin_amdgcn_readfirstlane(g_next);
+        if (li + stride < nrows) g_next = rows[li + stride];
         const TaskDev &TK = tasks[task_of(tasks, nt, g)];
         const int32_t r = g - TK.row_base;
         const TR_GLOBAL int32_t *__restrict__ ai = as_global(TK.ai);
@@ -657,9 +672,12 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
         const TR_GLOBAL int32_t *__restrict__ bp = as_global(TK.bp);
         const TR_GLOBAL int32_t *__restrict__ bi = as_global(TK.bi);
         const TR_GLOBAL double *__restrict__ bv = as_global(TK.bv);
+        const int32_t a0 = as_global(TK.ap)[r], a1 = as_global(TK.ap)[r + 1];
+        const int64_t rs = rowptr[g];
+        const int64_t re = rowptr[g + 1];
+        const Panel P = find_panel(panels, npanels, g);
         for (int t = l; t < T; t += kWave) keys[t] = kEmpty;
         wave_sync();
-        const int32_t a0 = as_global(TK.ap)[r], a1 = as_global(TK.ap)[r + 1];
         for (int32_t pb = a0; pb < a1; pb += kWave) {
             int32_t bs = 0, be = 0;
             double a = 0.0;
@@ -716,15 +734,47 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
                             }
                         }
                         wave_sync();
-                        while (ballot(pend)) {
-                            if (pend) atomicMin(&own[h], (uint32_t)l);
+                        if constexpr (kMask) {
+                            // each slot's lanes as a mask; its lowest lane
+                            // applies them all in lane order, fetching the
+                            // other lanes' operands across the wave
+                            if (pend) atomicOr(&own[h], 1ull << l);
                             wave_sync();
-                            if (pend && own[h] == (uint32_t)l) {
-                                vals[h] = __fma_rn(at, b, vals[h]);
-                                own[h] = kWave;
-                                pend = false;
+                            unsigned long long rem = 0ull;
+                            double v = 0.0;
+                            if (pend) {
+                                const unsigned long long m = own[h];
+                                if (__ffsll(m) - 1 == l) {
+                                    rem = m;
+                                    v = vals[h];
+                                }
+                            }
+                            while (ballot(rem != 0ull)) {  // every lane takes part in the fetches
+                                const int t = rem ? __ffsll(rem) - 1 : l;
+                                const double ta = __shfl(at, t, kWave);
+                                const double tb = __shfl(b, t, kWave);
+                                if (rem) {
+                                    v = __fma_rn(ta, tb, v);
+                                    rem &= rem - 1ull;
+                                }
                             }
                             wave_sync();
+                            if (pend && (__ffsll(own[h]) - 1 == l)) {
+                                vals[h] = v;
+                                own[h] = 0ull;
+                            }
+                            wave_sync();
+                        } else {
+                            while (ballot(pend)) {
+                                if (pend) atomicMin(&own[h], (uint32_t)l);
+                                wave_sync();
+                                if (pend && own[h] == (uint32_t)l) {
+                                    vals[h] = __fma_rn(at, b, vals[h]);
+                                    own[h] = kWave;
+                                    pend = false;
+                                }
+                                wave_sync();
+                            }
                         }
                     }
                     continue;
@@ -778,11 +828,9 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__rest
         wave_sync();
         const int n = wave_compact_lds<double>(keys, vals, T);
         wave_sort_lds<double>(keys, vals, n);
-        const int64_t rs = rowptr[g];
-        const int expect = (int)(rowptr[g + 1] - rs);
+        const int expect = (int)(re - rs);
         if (l == 0 && n != expect) atomicOr(&cnt->overflow, 2);
         const int nw = n < expect ? n : expect;  // never write past the row
-        const Panel &P = find_panel(panels, npanels, g);
         TR_GLOBAL int32_t *oi = as_global(P.idx) + (rs - P.nnz_begin);
         TR_GLOBAL double *ov = as_global(P.val) + (rs - P.nnz_begin);
         for (int t = l; t < nw; t += kWave) {
