@@ -210,10 +210,14 @@ def test_galerkin_operators(ctx):
     run(ctx, gen.transpose(P), AP, "P^T*(A*P)")
 
 
-def test_light_short_rows_ordered(ctx):
-    # 64 A entries per row over B rows of 3 entries in 16 columns: every
-    # round of 64 flattened products meets a column about 12 times, and the
-    # values span 2^-20..2^20, so any application out of A-order changes bits
+@pytest.mark.parametrize("per_row", [64, 40, 20], ids=["owner512", "mask256", "mask128"])
+def test_light_short_rows_ordered(ctx, per_row):
+    # per_row A entries per row over B rows of 3 entries in 16 columns: a
+    # round of up to 64 flattened products meets each column several times,
+    # and the values span 2^-20..2^20, so any application out of A-order
+    # changes bits.
+    # 192 products per row take 512-slot tables (per-slot owner rounds); 120
+    # and 60 take 256- and 128-slot tables (per-slot lane masks)
     rng = np.random.default_rng(17)
     k, ncols = 400, 16
     bl = [np.sort(rng.choice(ncols, size=3, replace=False)) for _ in range(k)]
@@ -221,7 +225,7 @@ def test_light_short_rows_ordered(ctx):
     bval = rng.uniform(-1, 1, bptr[-1]) * np.exp2(rng.integers(-20, 21, bptr[-1]))
     B = gen.from_arrays(k, ncols, bptr, np.concatenate(bl), bval)
     m = 2000
-    al = [np.sort(rng.choice(k, size=64, replace=False)) for _ in range(m)]
+    al = [np.sort(rng.choice(k, size=per_row, replace=False)) for _ in range(m)]
     aptr = np.concatenate([[0], np.cumsum([len(x) for x in al])])
     aval = rng.uniform(-1, 1, aptr[-1]) * np.exp2(rng.integers(-20, 21, aptr[-1]))
     A = gen.from_arrays(m, k, aptr, np.concatenate(al), aval)
