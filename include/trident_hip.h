@@ -133,7 +133,8 @@ int tr_spgemm_flops(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t 
  *    heavy (column-windowed) path, [64, 1024];
  *  - window_entries: heavy window packing quantum H (a window holds at most
  *    H + 127 entries), [64, 3 * heavy_table / 4 - 127];
- *  - item_windows: heavy windows per work item, [1, 64];
+ *  - item_windows: heavy windows per work item, [1, 64] (rows of more than
+ *    2048 entries use a quarter of it, at least 1);
  *  - heavy_table: slots of the heavy numeric table, 2048 (256-thread
  *    workgroups) or 4096 (512-thread workgroups).
  * Results do not depend on the tuning (bit-identical), only the time does.

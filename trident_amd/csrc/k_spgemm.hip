@@ -472,8 +472,12 @@ __global__ __launch_bounds__(NTH) void k_sym_heavy(
 #pragma unroll
             for (int u = 0; u < kPerThread; ++u)
                 if (flags & (1 << u)) first_lds[rk++] = tid * kPerThread + u;
+            // rows of more than kLongRow entries get a quarter of the
+            // windows per item: their items are the longest, and the last
+            // of them would otherwise run alone at the end of the numeric
+            const int32_t iw = (a1 - a0 > kLongRow) ? max(1, item_windows / 4) : item_windows;
             if (tid == 0) {
-                const int32_t nitems = (nwin + item_windows - 1) / item_windows;
+                const int32_t nitems = (nwin + iw - 1) / iw;
                 const int32_t wb = atomicAdd(&cnt->windows, nwin);
                 const int32_t ib = atomicAdd(&cnt->items, nitems);
                 if (wb + nwin > win_cap || ib + nitems > item_cap) {
@@ -484,8 +488,8 @@ __global__ __launch_bounds__(NTH) void k_sym_heavy(
                     for (int32_t it = 0; it < nitems; ++it) {
                         Item I;
                         I.row = g;
-                        I.wbeg = wb + it * item_windows;
-                        I.wend = min(wb + nwin, I.wbeg + item_windows);
+                        I.wbeg = wb + it * iw;
+                        I.wend = min(wb + nwin, I.wbeg + iw);
                         I.pad = 0;
                         items[ib + it] = I;
                     }
@@ -1421,7 +1425,7 @@ void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool captu
             const int64_t wsplits = (int64_t)max_n / kWinMaxWidth + 1;
             const int64_t q = std::min<int64_t>(pl.tune.window_entries, kSparseH);
             const int64_t wcap = (int64_t)(pl.h.heavy_flops / q) + (int64_t)nh * (2 * passes + wsplits) + 16;
-            const int64_t icap = wcap / pl.tune.item_windows + (int64_t)nh * passes + 16;
+            const int64_t icap = wcap / std::max(1, pl.tune.item_windows / 4) + (int64_t)nh * passes + 16;
             TR_REQUIRE(wcap < INT32_MAX && icap < INT32_MAX, "spgemm: heavy work list exceeds int32 capacity");
             pl.win_cap = (int32_t)wcap;
             pl.item_cap = (int32_t)icap;

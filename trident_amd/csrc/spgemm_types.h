@@ -30,6 +30,7 @@ static_assert(kWinMaxWidth <= (1 << 20) && kPassBits <= 20, "window column offse
 constexpr int kSparseAvg = 192;    // a pass with <= this many entries per width quantum is sparse
 constexpr int kSparseH = 256;      // sparse windows: count quantum (<= kSparseH + 127 entries, any width)
 constexpr int kHeavyThreads = 1024;  // symbolic heavy block
+constexpr int kLongRow = 2048;       // heavy rows with more A entries take shorter items
 
 struct Window {
     int32_t c0, c1;   // column range [c0, c1)
