@@ -107,18 +107,35 @@ __device__ __forceinline__ void stream_cols(const TR_GLOBAL int32_t *__restrict_
 // Same visit with 16-byte loads for the aligned body of the segment (four
 // columns per lane per load, four loads in flight), scalar head and tail.
 // Alignment is taken from the address: caller tiles may place indices at any
-// 4-byte offset.
-template <typename F>
+// 4-byte offset.  kEager issues the head, the tail and the first two groups
+// per lane at once (more registers: for kernels whose occupancy LDS bounds).
+template <bool kEager, typename F>
 __device__ __forceinline__ void stream_cols_v4(const TR_GLOBAL int32_t *__restrict__ bi, int32_t s, int32_t e,
                                                F &&f) {
     const int l = lane_id();
     const int32_t mis = (int32_t)(((uintptr_t)(bi + s) >> 2) & 3);  // dwords past a 16-byte boundary
     const int32_t sa = min(e, s + ((4 - mis) & 3));                  // first 16-byte aligned index
-    if (s + l < sa) f(bi[s + l]);
     const int32_t nv = (e - sa) >> 2;                                // whole int4 groups
     typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
     const TR_GLOBAL i32x4 *__restrict__ b4 = (const TR_GLOBAL i32x4 *)(bi + sa);
-    int32_t v = l;
+    const int32_t ta = sa + (nv << 2);
+    if constexpr (kEager) {
+    // head, tail and the first two groups per lane are loaded together: a
+    // segment of up to 2 * 64 groups costs one round trip
+    const bool hh = s + l < sa, ht = ta + l < e, h0 = l < nv, h1 = l + kWave < nv;
+    int32_t jh = 0, jt = 0;
+    i32x4 g0 = {0, 0, 0, 0}, g1 = {0, 0, 0, 0};
+    if (hh) jh = bi[s + l];
+    if (ht) jt = bi[ta + l];
+    if (h0) g0 = b4[l];
+    if (h1) g1 = b4[l + kWave];
+    if (hh) f(jh);
+    if (ht) f(jt);
+    if (h0) f(g0.x), f(g0.y), f(g0.z), f(g0.w);
+    if (h1) f(g1.x), f(g1.y), f(g1.z), f(g1.w);
+    }
+    int32_t v = kEager ? l + 2 * kWave : l;
+    if (!kEager && s + l < sa) f(bi[s + l]);
     for (; v + 3 * kWave < nv; v += 4 * kWave) {
         const i32x4 x0 = b4[v], x1 = b4[v + kWave], x2 = b4[v + 2 * kWave], x3 = b4[v + 3 * kWave];
         f(x0.x), f(x0.y), f(x0.z), f(x0.w);
@@ -130,8 +147,7 @@ __device__ __forceinline__ void stream_cols_v4(const TR_GLOBAL int32_t *__restri
         const i32x4 x = b4[v];
         f(x.x), f(x.y), f(x.z), f(x.w);
     }
-    const int32_t ta = sa + (nv << 2);
-    if (ta + l < e) f(bi[ta + l]);
+    if (!kEager && ta + l < e) f(bi[ta + l]);
 }
 
 // Same with the values (all entries of one B row have distinct columns, so
@@ -343,7 +359,11 @@ __global__ __launch_bounds__(256) void k_sym_light(const TaskDev *__restrict__ t
 // the row's column windows (<= win_h + 127 entries, <= kWinMaxWidth columns)
 // appended to the device work lists.
 template <int PB, int NTH>
-__global__ __launch_bounds__(NTH) void k_sym_heavy(
+// Narrow bitmaps (2^18 bits on 512 threads, 2^19 on 1024) leave LDS for 32
+// waves per CU; the register budget is held to that too (64 VGPRs and a
+// few spilled bytes per lane): Galerkin 128^3 12.5 -> 12.0 ms, R-MAT s19
+// (2^19 columns) 17.8 -> 15.0 ms.
+__global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 1) void k_sym_heavy(
     const TaskDev *__restrict__ tasks, int nt, const int32_t *__restrict__ rows, Counters *__restrict__ cnt, int cls,
     int64_t *__restrict__ rownnz, int emit, int32_t win_h, int32_t item_windows, Window *__restrict__ wins,
     int32_t win_cap, Item *__restrict__ items, int32_t item_cap) {
@@ -352,6 +372,10 @@ __global__ __launch_bounds__(NTH) void k_sym_heavy(
     constexpr int kPerThread = kFine / NTH;        // consecutive fine blocks per thread: 2..8
     constexpr int kWavesB = NTH / kWave;
     static_assert(kPerThread >= 1 && kPerThread <= 8, "fine blocks per thread");
+    // B rows are streamed with eager loads where the bitmap's LDS already
+    // limits the kernel to four waves per SIMD (2^20 bits; 2^19 bits on 512
+    // threads), so the extra registers cost no occupancy
+    constexpr bool kEagerB = PB == 20 || (PB == 19 && NTH == 512);
     __shared__ uint32_t bm[kWords];
     __shared__ int32_t wsum[kWavesB];
     __shared__ int32_t sh_wbase;
@@ -391,7 +415,7 @@ __global__ __launch_bounds__(NTH) void k_sym_heavy(
                 const int nk = rem <= 0 ? 0 : min(kWave, (rem + kWavesB - 1) / kWavesB);
                 for (int t = 0; t < nk; ++t) {
                     const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
-                    stream_cols_v4(bi, s, e, [&](int32_t j) {
+                    stream_cols_v4<kEagerB>(bi, s, e, [&](int32_t j) {
                         const int64_t jj = (int64_t)j - base;
                         if (jj >= 0 && jj < width) atomicOr(&bm[jj >> 5], 1u << (jj & 31));
                     });
