@@ -658,7 +658,10 @@ constexpr int kLightPre = 4;  // B rows prefetched together per batch of a light
 // WPB waves per block, one row per wave (big tables use one-wave blocks so
 // that LDS, not the block shape, bounds the waves per CU)
 template <int T, int WPB>
-__global__ __launch_bounds__(WPB * kWave) void k_num_light(const TaskDev *__restrict__ tasks, int nt,
+// Tables of <= 256 slots leave LDS for 8 waves per SIMD; the registers are
+// held to that too (63 VGPRs, 12 spilled bytes per lane; about 1% on stage
+// 22:3 and Galerkin 128^3 against 65 VGPRs and 7 waves).
+__global__ __launch_bounds__(WPB * kWave, T <= 256 ? 8 : 1) void k_num_light(const TaskDev *__restrict__ tasks, int nt,
                                                            const int32_t *__restrict__ rows,
                                                            Counters *__restrict__ cnt, int cls,
                                                            const int64_t *__restrict__ rowptr,
