@@ -373,8 +373,8 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
     constexpr int kWavesB = NTH / kWave;
     static_assert(kPerThread >= 1 && kPerThread <= 8, "fine blocks per thread");
     // B rows are streamed with eager loads where the bitmap's LDS already
-    // limits the kernel to four waves per SIMD (2^20 bits; 2^19 bits on 512
-    // threads), so the extra registers cost no occupancy
+    // limits the kernel to four waves per SIMD (2^20 bits), so the extra
+    // registers cost no occupancy
     constexpr bool kEagerB = PB == 20 || (PB == 19 && NTH == 512);
     __shared__ uint32_t bm[kWords];
     __shared__ int32_t wsum[kWavesB];
@@ -1475,7 +1475,7 @@ void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool captu
         }
         if (pl.h.cls[kMidClass]) {
             if (pb == 18) launch_sym_heavy<18, kMidThreads>(ctx, pl, kMidClass, 64, emit);
-            else launch_sym_heavy<kMidPassBits, kMidThreads>(ctx, pl, kMidClass, 64, emit);
+            else launch_sym_heavy<kMidPassBits, kHeavyThreads>(ctx, pl, kMidClass, 32, emit);
         }
     }
     launch_exclusive_scan_i64(ctx, pl.rownnz, pl.rowptr, pl.rows_total, pl.scan_tmp);

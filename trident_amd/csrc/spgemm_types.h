@@ -17,7 +17,7 @@ constexpr int kHeavyClass = kNumLight;      // heavy rows with more than kMidMax
 constexpr int kMidClass = kNumLight + 1;    // heavy rows up to kMidMaxF multiplications
 constexpr int kNumClasses = kNumLight + 2;
 constexpr int kMidMaxF = 16384;
-constexpr int kMidPassBits = 19;            // mid rows: 2^19-column passes, two blocks per CU
+constexpr int kMidPassBits = 19;            // mid rows: at most 2^19-column passes, two blocks per CU
 constexpr int kMidThreads = 512;
 constexpr int kPassBits = 20;      // heavy bitmap covers 2^20 columns per pass
 constexpr int kFineShift = 7;      // fine block = 128 columns
