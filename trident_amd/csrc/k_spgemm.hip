@@ -354,8 +354,8 @@ __global__ __launch_bounds__(256) void k_sym_light(const TaskDev *__restrict__ t
 }
 
 // ---------------------------------------------------------------- symbolic, heavy rows
-// One 1024-thread block per row (grid-stride over the heavy list): a bitmap
-// of 2^20 columns in LDS per pass, popcounts per 128-column fine block, and
+// One NTH-thread block per row (grid-stride over the heavy list): a bitmap
+// of 2^PB columns in LDS per pass, popcounts per 128-column fine block, and
 // the row's column windows (<= win_h + 127 entries, <= kWinMaxWidth columns)
 // appended to the device work lists.
 template <int PB, int NTH>
