@@ -4,6 +4,7 @@
 //
 // Callers keep the same calls:
 //     LocalMatrix<int32_t,int32_t,double> A(csx_a), B(csx_b), C;
+//     LocalMatrix<int32_t,int32_t,double> At(handle, csc_a, conv_buffs);  // CSC -> CSR
 //     auto P = LocalMatrix<...>::spgemm(A, B);   // C = A*B
 //     LocalMatrix<...>::sp_mma(A, B, C);          // C += A*B
 //     LocalMatrix<...>::spadd(A, C);              // C += A
@@ -31,6 +32,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "CsxBuffers.hpp"
 #include "KokkosWrapTriple.hpp"
 #include "mmio_csx.hpp"
 #include "trident_hip.h"
@@ -38,51 +40,6 @@
 using MajorDim = mmio::MajorDim;
 
 namespace KokkosWrap {
-
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-namespace detail {
-
-struct ThreadCtx {
-    tr_ctx_t ctx = nullptr;
-    int device = 0;
-    ~ThreadCtx() {
-        if (ctx) tr_ctx_destroy(ctx);
-    }
-};
-
-inline ThreadCtx &thread_ctx() {
-    static thread_local ThreadCtx tc;
-    return tc;
-}
-
-inline tr_ctx_t ctx() {
-    ThreadCtx &tc = thread_ctx();
-    if (!tc.ctx) {
-        int rc = tr_ctx_create(&tc.ctx, tc.device, nullptr);
-        if (rc != TR_OK) throw Error(rc, "KokkosWrap: tr_ctx_create failed");
-    }
-    return tc.ctx;
-}
-
-inline void check(int rc, const char *what) {
-    if (rc != TR_OK) throw Error(rc, std::string(what) + ": " + tr_last_error(ctx()));
-}
-
-}  // namespace detail
-
-// Select the GPU of the calling thread's context (before its first use).
-inline void set_device(int device) {
-    detail::ThreadCtx &tc = detail::thread_ctx();
-    if (tc.ctx) throw Error(TR_ERR_INVALID, "KokkosWrap::set_device: context already created on this thread");
-    tc.device = device;
-}
-
-// The calling thread's library context (for direct C-ABI calls).
-inline tr_ctx_t context() { return detail::ctx(); }
 
 template <typename KIT, typename DIT, typename VT>
 struct LocalMatrix {
@@ -109,6 +66,20 @@ struct LocalMatrix {
             tr_csx_t csr;
             detail::check(tr_csc_to_csr(detail::ctx(), &storage, &csr), "LocalMatrix(csc)");
             storage = csr;
+        }
+    }
+
+    // View over a caller-owned device CSR; a CSC tile is converted to CSR,
+    // into `buffs` when given (no allocation once they are large enough),
+    // as the reference's constructor does (KokkosWrap.hpp:210-237).  The
+    // handle is accepted for the call-site shape and not used.
+    template <typename H>
+    LocalMatrix(H &handle, mmio::CSX<DIT, VT> *csx, CsxBuffers<DIT, VT> *buffs = nullptr)
+        : initialized(true), storage{} {
+        if (csx->majordim == MajorDim::ROWS) {
+            *this = LocalMatrix(csx);
+        } else {
+            storage = csc_to_csr(handle, csx, buffs);
         }
     }
 
@@ -167,6 +138,35 @@ struct LocalMatrix {
         c->buf = (char *)storage.buf;
         c->buf_size = storage.buf_size;
         return c;
+    }
+
+    // CSR of a CSC tile (KokkosWrap.hpp:441-531).  With `buffers` the result
+    // lives in them (grown with ensure(nnz, nrows + 1, ncols) like the
+    // reference) and stays borrowed; without, it is library-owned storage
+    // that freeBuffers() releases.
+    template <typename H>
+    tr_csx_t csc_to_csr(H &, mmio::CSX<DIT, VT> *csc, CsxBuffers<DIT, VT> *buffers = nullptr) {
+        if (csc->majordim != MajorDim::COLS)
+            throw KokkosWrap::Error(TR_ERR_INVALID, "LocalMatrix::csc_to_csr: input is not CSC");
+        tr_csx_t in{};
+        in.majordim = TR_COLS;
+        in.nrows = csc->nrows;
+        in.ncols = csc->ncols;
+        in.nnz = csc->nnz;
+        in.ptr_vec = (int32_t *)csc->ptr_vec;
+        in.idx_vec = (int32_t *)csc->idx_vec;
+        in.val = (double *)csc->val;
+        tr_csx_t out{};
+        if (buffers == nullptr) {
+            detail::check(tr_csc_to_csr(detail::ctx(), &in, &out), "LocalMatrix::csc_to_csr");
+            return out;
+        }
+        buffers->ensure((uint64_t)csc->nnz, (uint64_t)csc->nrows + 1, (uint64_t)csc->ncols);
+        detail::check(tr_csc_to_csr_into(detail::ctx(), &in, buffers->d_node_rowptrs, buffers->d_node_colinds,
+                                         buffers->d_node_vals),
+                      "LocalMatrix::csc_to_csr");
+        out = buffers->view();
+        return out;
     }
 
     // Release library-owned storage (KokkosWrap.hpp:294-302); borrowed views

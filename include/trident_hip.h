@@ -114,6 +114,19 @@ int tr_sp_mma(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *C_in
  * A.nnz == 0: no-op; C uninitialised: C := copy of A. */
 int tr_spadd(tr_ctx_t ctx, const tr_csx_t *A, tr_csx_t *C_inout);
 
+/* out = A + B (both CSR with sorted unique columns, same shape) into
+ * caller-owned arrays: out->ptr_vec (nrows + 1 entries), out->idx_vec and
+ * out->val (room for `capacity` entries); on success out->nnz and the
+ * shape fields are set.  *nnz_needed receives nnz(A + B); if it exceeds
+ * `capacity` nothing is written and TR_ERR_OVERFLOW is returned, so the
+ * caller can grow its buffers and call again.  The output must not alias
+ * an operand.  Replaces cusparse_spgeam (include/cusparse_helpers.cuh:
+ * 799-905), which grows C_accum's CsxBuffers between its nnz and numeric
+ * calls (:866-867); this is the double-buffer add of sp_mma_hybrid
+ * (:708-739). */
+int tr_spadd_into(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *out, int64_t capacity,
+                  int64_t *nnz_needed);
+
 /* Exact nnz(C) and the device bytes tr_spgemm needs for C = A*B (output tile
  * plus peak scratch): the admission bound used by the serving loop.  No
  * counterpart in the reference (its buffers grow on demand,
@@ -251,6 +264,13 @@ int tr_csx_transpose(tr_ctx_t ctx, const tr_csx_t *M, tr_csx_t *out);
 /* CSC tile -> CSR tile of the same matrix (majordim TR_COLS in, TR_ROWS out). */
 int tr_csc_to_csr(tr_ctx_t ctx, const tr_csx_t *M_csc, tr_csx_t *out);
 
+/* CSC tile -> CSR tile of the same matrix into caller-owned arrays
+ * (ptr: nrows + 1, idx/val: nnz entries), rows sorted.  Replaces
+ * cusparse_csc_to_csr with conversion buffers (include/cusparse_helpers.cuh:
+ * 607-705) and LocalMatrix::csc_to_csr(handle, csx, buffs)
+ * (include/KokkosWrap.hpp:441-531). */
+int tr_csc_to_csr_into(tr_ctx_t ctx, const tr_csx_t *M_csc, int32_t *ptr, int32_t *idx, double *val);
+
 /* Sort the columns of every row in place (values follow). */
 int tr_csx_sort_rows(tr_ctx_t ctx, tr_csx_t *M);
 
@@ -279,6 +299,30 @@ int tr_free(tr_ctx_t ctx, void *ptr);
 int tr_memcpy_h2d(tr_ctx_t ctx, void *dst, const void *src, size_t bytes);
 int tr_memcpy_d2h(tr_ctx_t ctx, void *dst, const void *src, size_t bytes);
 int tr_memcpy_d2d(tr_ctx_t ctx, void *dst, const void *src, size_t bytes);
+/* Fill `bytes` bytes at dst with `value`, ordered on the context's stream
+ * (the cudaMemset of the reference's buffers, include/utils.cuh:569). */
+int tr_memset(tr_ctx_t ctx, void *dst, int value, size_t bytes);
+
+/* Context-free device allocation on `device` (thread-safe, outside any
+ * context's pool): storage that outlives or is shared between the contexts
+ * of several host threads, such as the CsxBuffers of the partial C that the
+ * accumulation thread grows (include/utils.cuh:527-784, whose ensure()
+ * reallocates with cudaMalloc, :637-667). */
+int tr_device_malloc(int device, void **ptr, size_t bytes);
+int tr_device_free(void *ptr);
+
+/* Cross-stream ordering between contexts: tr_event_record marks the work
+ * queued so far on ctx's stream; after tr_ctx_wait_event, work queued on
+ * ctx (another context) waits for the marked work.  This is how a product
+ * made on the main thread's stream is handed to the accumulation thread's
+ * stream (AccumThreadHandle, include/hns_spgemm.cuh:32-130). */
+typedef struct tr_event *tr_event_t;
+int tr_event_create(tr_event_t *ev);
+int tr_event_record(tr_ctx_t ctx, tr_event_t ev);
+int tr_ctx_wait_event(tr_ctx_t ctx, tr_event_t ev);
+int tr_event_synchronize(tr_event_t ev);
+int tr_event_destroy(tr_event_t ev);
+
 /* Return cached, unused pool blocks to the device. */
 int tr_pool_trim(tr_ctx_t ctx);
 /* Bytes currently handed out / cached by the pool. */

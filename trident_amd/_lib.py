@@ -74,6 +74,7 @@ SIGNATURES = [
     ("tr_spgemm_panels", ctypes.c_int, [_vp, _csx, _csx, _i64, _csx, _P(_i32), _i32, _P(_i32)]),
     ("tr_sp_mma", ctypes.c_int, [_vp, _csx, _csx, _csx]),
     ("tr_spadd", ctypes.c_int, [_vp, _csx, _csx]),
+    ("tr_spadd_into", ctypes.c_int, [_vp, _csx, _csx, _csx, _i64, _P(_i64)]),
     ("tr_symbolic", ctypes.c_int, [_vp, _csx, _csx, _P(_i64), _P(_sz)]),
     ("tr_spgemm_flops", ctypes.c_int, [_vp, _csx, _csx, _P(_i64)]),
     ("tr_tuning_default", ctypes.c_int, [_P(tr_tuning_t)]),
@@ -99,6 +100,7 @@ SIGNATURES = [
     ("tr_compress", ctypes.c_int, [_vp, _csx, _vp, _csx]),
     ("tr_csx_transpose", ctypes.c_int, [_vp, _csx, _csx]),
     ("tr_csc_to_csr", ctypes.c_int, [_vp, _csx, _csx]),
+    ("tr_csc_to_csr_into", ctypes.c_int, [_vp, _csx, _vp, _vp, _vp]),
     ("tr_csx_sort_rows", ctypes.c_int, [_vp, _csx]),
     ("tr_csx_validate", ctypes.c_int, [_vp, _csx, _P(_i32)]),
     ("tr_csx_free", ctypes.c_int, [_vp, _csx]),
@@ -109,6 +111,14 @@ SIGNATURES = [
     ("tr_memcpy_h2d", ctypes.c_int, [_vp, _vp, _vp, _sz]),
     ("tr_memcpy_d2h", ctypes.c_int, [_vp, _vp, _vp, _sz]),
     ("tr_memcpy_d2d", ctypes.c_int, [_vp, _vp, _vp, _sz]),
+    ("tr_memset", ctypes.c_int, [_vp, _vp, ctypes.c_int, _sz]),
+    ("tr_device_malloc", ctypes.c_int, [ctypes.c_int, _P(_vp), _sz]),
+    ("tr_device_free", ctypes.c_int, [_vp]),
+    ("tr_event_create", ctypes.c_int, [_P(_vp)]),
+    ("tr_event_record", ctypes.c_int, [_vp, _vp]),
+    ("tr_ctx_wait_event", ctypes.c_int, [_vp, _vp]),
+    ("tr_event_synchronize", ctypes.c_int, [_vp]),
+    ("tr_event_destroy", ctypes.c_int, [_vp]),
     ("tr_pool_trim", ctypes.c_int, [_vp]),
     ("tr_pool_stats", ctypes.c_int, [_vp, _P(_sz), _P(_sz)]),
     ("tr_pool_set_limit", ctypes.c_int, [_vp, _sz]),

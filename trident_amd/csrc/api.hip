@@ -429,6 +429,22 @@ int tr_sp_mma(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *C) {
     });
 }
 
+// Both operands of a sorted add: CSR, same shape, well formed, sorted rows.
+static void require_add_operands(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, const char *what) {
+    const std::string w(what);
+    check_csx(C, "C");
+    TR_REQUIRE(A->majordim == TR_ROWS && C->majordim == TR_ROWS, w + ": operands must be CSR");
+    TR_REQUIRE(A->nrows == C->nrows && A->ncols == C->ncols, w + ": shape mismatch");
+    int32_t *st = (int32_t *)scratch(ctx, S_PANELS, 64);
+    TR_HIP(hipMemsetAsync(st, 0, 2 * sizeof(int32_t), ctx->stream));
+    launch_validate(ctx, A, st);
+    launch_validate(ctx, C, st + 1);
+    int32_t hs[2];
+    read_array(ctx, hs, st, sizeof(hs));
+    if ((hs[0] | hs[1]) & 3) fail(TR_ERR_INVALID, w + ": malformed CSR operand");
+    if ((hs[0] | hs[1]) & 4) fail(TR_ERR_INVALID, w + ": operands need sorted unique columns");
+}
+
 int tr_spadd(tr_ctx_t ctx, const tr_csx_t *A, tr_csx_t *C) {
     TR_CTX_REQUIRED(ctx);
     return guarded(ctx, [&] {
@@ -439,21 +455,35 @@ int tr_spadd(tr_ctx_t ctx, const tr_csx_t *A, tr_csx_t *C) {
             copy_tile(ctx, A, C);  // KokkosWrap.hpp:403-407 (deep copy here)
             return;
         }
-        check_csx(C, "C");
-        TR_REQUIRE(A->majordim == TR_ROWS && C->majordim == TR_ROWS, "tr_spadd: operands must be CSR");
-        TR_REQUIRE(A->nrows == C->nrows && A->ncols == C->ncols, "tr_spadd: shape mismatch");
-        int32_t *st = (int32_t *)scratch(ctx, S_PANELS, 64);
-        TR_HIP(hipMemsetAsync(st, 0, 2 * sizeof(int32_t), ctx->stream));
-        launch_validate(ctx, A, st);
-        launch_validate(ctx, C, st + 1);
-        int32_t hs[2];
-        read_array(ctx, hs, st, sizeof(hs));
-        if ((hs[0] | hs[1]) & 3) fail(TR_ERR_INVALID, "tr_spadd: malformed CSR operand");
-        if ((hs[0] | hs[1]) & 4) fail(TR_ERR_INVALID, "tr_spadd: operands need sorted unique columns");
+        require_add_operands(ctx, A, C, "tr_spadd");
         tr_csx_t acc;
         spadd(ctx, A, C, &acc);
         if (ctx_owned(ctx, C)) release_csx(ctx, C);
         *C = acc;
+    });
+}
+
+int tr_spadd_into(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *out, int64_t capacity,
+                  int64_t *nnz_needed) {
+    TR_CTX_REQUIRED(ctx);
+    return guarded(ctx, [&] {
+        TR_REQUIRE(out != nullptr && nnz_needed != nullptr, "tr_spadd_into: null output");
+        check_csx(A, "A");
+        require_add_operands(ctx, A, B, "tr_spadd_into");
+        TR_REQUIRE(capacity >= 0 && capacity <= INT32_MAX, "tr_spadd_into: capacity must be in [0, INT32_MAX]");
+        TR_REQUIRE(out->ptr_vec != nullptr && (capacity == 0 || (out->idx_vec && out->val)),
+                   "tr_spadd_into: output arrays missing");
+        for (const tr_csx_t *in : {A, B})
+            TR_REQUIRE(out->ptr_vec != in->ptr_vec && out->idx_vec != in->idx_vec && out->val != in->val,
+                       "tr_spadd_into: output aliases an operand");
+        const int64_t nnz = spadd_into(ctx, A, B, out->ptr_vec, out->idx_vec, out->val, capacity);
+        *nnz_needed = nnz;
+        if (nnz > capacity) fail(TR_ERR_OVERFLOW, "tr_spadd_into: result needs " + std::to_string(nnz) +
+                                                      " entries, capacity " + std::to_string(capacity));
+        out->majordim = TR_ROWS;
+        out->nrows = A->nrows;
+        out->ncols = A->ncols;
+        out->nnz = nnz;
     });
 }
 
@@ -558,6 +588,28 @@ int tr_csc_to_csr(tr_ctx_t ctx, const tr_csx_t *M, tr_csx_t *out) {
     });
 }
 
+int tr_csc_to_csr_into(tr_ctx_t ctx, const tr_csx_t *M, int32_t *ptr, int32_t *idx, double *val) {
+    TR_CTX_REQUIRED(ctx);
+    return guarded(ctx, [&] {
+        check_csx(M, "M");
+        TR_REQUIRE(M->majordim == TR_COLS, "tr_csc_to_csr_into: input must be CSC (TR_COLS)");
+        TR_REQUIRE(ptr != nullptr && (M->nnz == 0 || (idx && val)), "tr_csc_to_csr_into: null output");
+        TR_REQUIRE(ptr != M->ptr_vec && (M->nnz == 0 || (idx != M->idx_vec && val != M->val)),
+                   "tr_csc_to_csr_into: output aliases the input");
+        require_valid(ctx, M, "tr_csc_to_csr_into", false);
+        tr_csx_t t;
+        std::memset(&t, 0, sizeof(t));
+        t.majordim = TR_ROWS;
+        t.nrows = M->nrows;
+        t.ncols = M->ncols;
+        t.nnz = M->nnz;
+        t.ptr_vec = ptr;
+        t.idx_vec = idx;
+        t.val = val;
+        launch_transpose(ctx, M, &t);
+    });
+}
+
 int tr_csx_sort_rows(tr_ctx_t ctx, tr_csx_t *M) {
     TR_CTX_REQUIRED(ctx);
     return guarded(ctx, [&] {
@@ -658,6 +710,61 @@ int tr_memcpy_d2h(tr_ctx_t ctx, void *dst, const void *src, size_t bytes) {
 int tr_memcpy_d2d(tr_ctx_t ctx, void *dst, const void *src, size_t bytes) {
     TR_CTX_REQUIRED(ctx);
     return guarded(ctx, [&] { TR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream)); });
+}
+
+int tr_memset(tr_ctx_t ctx, void *dst, int value, size_t bytes) {
+    TR_CTX_REQUIRED(ctx);
+    return guarded(ctx, [&] {
+        TR_REQUIRE(dst != nullptr || bytes == 0, "tr_memset: null pointer");
+        if (bytes) TR_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    });
+}
+
+int tr_device_malloc(int device, void **ptr, size_t bytes) {
+    return guarded(nullptr, [&] {
+        TR_REQUIRE(ptr != nullptr, "tr_device_malloc: null output");
+        *ptr = nullptr;
+        TR_HIP(hipSetDevice(device));
+        TR_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    });
+}
+
+int tr_device_free(void *ptr) {
+    return guarded(nullptr, [&] {
+        if (ptr) TR_HIP(hipFree(ptr));
+    });
+}
+
+int tr_event_create(tr_event_t *ev) {
+    if (!ev) return TR_ERR_INVALID;
+    *ev = nullptr;
+    return guarded(nullptr, [&] {
+        hipEvent_t e;
+        TR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        *ev = (tr_event_t)e;
+    });
+}
+
+int tr_event_record(tr_ctx_t ctx, tr_event_t ev) {
+    TR_CTX_REQUIRED(ctx);
+    if (!ev) return TR_ERR_INVALID;
+    return guarded(ctx, [&] { TR_HIP(hipEventRecord((hipEvent_t)ev, ctx->stream)); });
+}
+
+int tr_ctx_wait_event(tr_ctx_t ctx, tr_event_t ev) {
+    TR_CTX_REQUIRED(ctx);
+    if (!ev) return TR_ERR_INVALID;
+    return guarded(ctx, [&] { TR_HIP(hipStreamWaitEvent(ctx->stream, (hipEvent_t)ev, 0)); });
+}
+
+int tr_event_synchronize(tr_event_t ev) {
+    if (!ev) return TR_ERR_INVALID;
+    return guarded(nullptr, [&] { TR_HIP(hipEventSynchronize((hipEvent_t)ev)); });
+}
+
+int tr_event_destroy(tr_event_t ev) {
+    if (!ev) return TR_OK;
+    return guarded(nullptr, [&] { TR_HIP(hipEventDestroy((hipEvent_t)ev)); });
 }
 
 int tr_pool_trim(tr_ctx_t ctx) {

@@ -218,7 +218,7 @@ void batch_compress(tr_batch *b) {
         // B: row segments
         BatchOperand &bc = b->Bc[t];
         if (B.nrows == 0) TR_LAUNCH(ctx, "zero_ptr", k_zero_i32, 1, 64, 0, bc.optr, (int64_t)1);
-        launch_compress_fixed(ctx, B.nrows, B.ptr_vec, B.idx_vec, B.val, mask, b->kept_tmp[t], b->np64[t],
+        launch_compress_fixed(ctx, B.nrows, B.ptr_vec, B.idx_vec, B.val, B.nnz, mask, b->kept_tmp[t], b->np64[t],
                               b->scan_tmp[t], bc.optr, bc.oidx, bc.oval);
     }
 }

@@ -42,11 +42,14 @@ __global__ void k_compress_count(const int32_t *__restrict__ ptr, int32_t n, con
     kept[i] = keep ? ptr[i + 1] - ptr[i] : 0;
 }
 
-// one wave per 64 consecutive segments; flattened over their entries
+// one wave per 64 consecutive segments; flattened over their entries.
+// Reads stay in [0, nnz_in) and writes in [0, cap) whatever the pointer
+// array holds (a malformed one gives a wrong tile, never a stray access).
 __global__ void k_compress_write(const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
                                  const double *__restrict__ val, int32_t n, const uint64_t *__restrict__ mask,
                                  const int64_t *__restrict__ newptr64, int32_t *__restrict__ optr,
-                                 int32_t *__restrict__ oidx, double *__restrict__ oval) {
+                                 int32_t *__restrict__ oidx, double *__restrict__ oval, int64_t nnz_in,
+                                 int64_t cap) {
     const int l = lane_id();
     const int64_t g = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     const int64_t s0 = g * kWave;
@@ -74,8 +77,10 @@ __global__ void k_compress_write(const int32_t *__restrict__ ptr, const int32_t 
         if (x < (int64_t)(last - gbeg) && keep_o) {
             int64_t p = (int64_t)gbeg + x;
             int64_t q = nb_o + (p - ob);
-            oidx[q] = idx[p];
-            oval[q] = val[p];
+            if (p >= 0 && p < nnz_in && q >= 0 && q < cap) {
+                oidx[q] = idx[p];
+                oval[q] = val[p];
+            }
         }
     }
 }
@@ -95,15 +100,15 @@ void launch_bitmask_and(tr_ctx *ctx, const uint64_t *a, const uint64_t *b, int32
 // Segment compression into caller-provided arrays with capacity for the
 // input's nnz; no host round trip (usable inside a captured graph).
 void launch_compress_fixed(tr_ctx *ctx, int32_t n, const int32_t *ptr, const int32_t *idx, const double *val,
-                           const uint64_t *mask, int32_t *kept_tmp, int64_t *np64, int64_t *scan_tmp, int32_t *optr,
-                           int32_t *oidx, double *oval) {
+                           int64_t nnz_in, const uint64_t *mask, int32_t *kept_tmp, int64_t *np64, int64_t *scan_tmp,
+                           int32_t *optr, int32_t *oidx, double *oval) {
     if (n <= 0) return;
     TR_LAUNCH(ctx, "compress_count", k_compress_count, div_up((uint64_t)n, 256), 256, 0, ptr, n, mask, kept_tmp);
     launch_exclusive_scan_i32_to_i64(ctx, kept_tmp, np64, n, scan_tmp);
     const int wpb = 4;
     const uint32_t groups = div_up((uint64_t)n, kWave);
     TR_LAUNCH(ctx, "compress_write", k_compress_write, div_up(groups, wpb), wpb * kWave, 0, ptr, idx, val, n, mask,
-              np64, optr, oidx, oval);
+              np64, optr, oidx, oval, nnz_in, nnz_in);
 }
 
 int64_t compress(tr_ctx *ctx, const tr_csx_t *m, const uint64_t *mask, tr_csx_t *out) {
@@ -121,7 +126,7 @@ int64_t compress(tr_ctx *ctx, const tr_csx_t *m, const uint64_t *mask, tr_csx_t 
     const int wpb = 4;
     uint32_t groups = div_up((uint64_t)n, kWave);
     TR_LAUNCH(ctx, "compress_write", k_compress_write, div_up(groups, wpb), wpb * kWave, 0, m->ptr_vec, m->idx_vec,
-              m->val, n, mask, np64, out->ptr_vec, out->idx_vec, out->val);
+              m->val, n, mask, np64, out->ptr_vec, out->idx_vec, out->val, m->nnz, nnz);
     return nnz;
 }
 

@@ -306,6 +306,30 @@ void add_write(tr_ctx *ctx, AddPlan &ap, int32_t *oi, double *ov) {
               ap.d, ap.nt, rows, ap.cstart, ap.crow, (int32_t *)nullptr, ap.coff, oi, ov);
 }
 
+// optr/oi/ov = A + C (both CSR, sorted) into caller arrays with room for
+// `capacity` entries; returns nnz(A + C) and writes nothing if it exceeds
+// the capacity.
+int64_t spadd_into(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, int32_t *optr, int32_t *oi, double *ov,
+                   int64_t capacity) {
+    Workspace ws(ctx, false);
+    AddPlan ap;
+    AddTaskDev t{};
+    t.xp = A->ptr_vec;
+    t.xi = A->idx_vec;
+    t.xv = A->val;
+    t.yp = C->ptr_vec;
+    t.yi = C->idx_vec;
+    t.yv = C->val;
+    t.m = A->nrows;
+    t.optr = optr;
+    add_setup(ctx, ws, ap, {t}, A->nnz, C->nnz);
+    add_count(ctx, ap, false);
+    const int64_t nnz = add_total(ctx, ap);
+    if (nnz > capacity) return nnz;
+    add_write(ctx, ap, oi, ov);
+    return nnz;
+}
+
 // out = A + C (both CSR, sorted), out context-owned.
 void spadd(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, tr_csx_t *out) {
     Workspace ws(ctx, false);

@@ -198,8 +198,8 @@ void launch_bitmask_gen(tr_ctx *ctx, const int32_t *ptr, int32_t n, uint64_t *ma
 void launch_bitmask_and(tr_ctx *ctx, const uint64_t *a, const uint64_t *b, int32_t words, uint64_t *out);
 int64_t compress(tr_ctx *ctx, const tr_csx_t *m, const uint64_t *mask, tr_csx_t *out);
 void launch_compress_fixed(tr_ctx *ctx, int32_t n, const int32_t *ptr, const int32_t *idx, const double *val,
-                           const uint64_t *mask, int32_t *kept_tmp, int64_t *np64, int64_t *scan_tmp, int32_t *optr,
-                           int32_t *oidx, double *oval);
+                           int64_t nnz_in, const uint64_t *mask, int32_t *kept_tmp, int64_t *np64, int64_t *scan_tmp,
+                           int32_t *optr, int32_t *oidx, double *oval);
 
 // k_spgemm.hip
 void validate_operands(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B);
@@ -218,6 +218,8 @@ int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max
 
 // k_spadd.hip
 void spadd(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, tr_csx_t *out);
+int64_t spadd_into(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, int32_t *optr, int32_t *oi, double *ov,
+                   int64_t capacity);
 
 // Read a device value through pinned staging (synchronises the stream).
 template <typename T>
