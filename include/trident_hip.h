@@ -127,6 +127,21 @@ int tr_spadd(tr_ctx_t ctx, const tr_csx_t *A, tr_csx_t *C_inout);
 int tr_spadd_into(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *out, int64_t capacity,
                   int64_t *nnz_needed);
 
+/* tr_spadd_into that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory while they run
+ * (tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as tr_spadd_into; its temporaries live in a fixed
+ * device array of TR_NOALLOC_SCRATCH_BYTES per GPU that the library holds
+ * until the process ends, and operands needing more (about
+ * 28 * nrows + (nnz(A) + nnz(B)) / 256 + max(nnz(A), nnz(B)) / 8 bytes) are
+ * rejected with TR_ERR_INVALID.  Calls on one GPU share that array: calls
+ * that overlap on one GPU (several streams or threads at once) race and
+ * must be serialised by the caller.  Like tr_spadd_into it synchronises the
+ * context's stream to read nnz(A + B). */
+#define TR_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int tr_spadd_into_noalloc(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *out, int64_t capacity,
+                          int64_t *nnz_needed);
+
 /* Exact nnz(C) and the device bytes tr_spgemm needs for C = A*B (output tile
  * plus peak scratch): the admission bound used by the serving loop.  No
  * counterpart in the reference (its buffers grow on demand,

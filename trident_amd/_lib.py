@@ -75,6 +75,7 @@ SIGNATURES = [
     ("tr_sp_mma", ctypes.c_int, [_vp, _csx, _csx, _csx]),
     ("tr_spadd", ctypes.c_int, [_vp, _csx, _csx]),
     ("tr_spadd_into", ctypes.c_int, [_vp, _csx, _csx, _csx, _i64, _P(_i64)]),
+    ("tr_spadd_into_noalloc", ctypes.c_int, [_vp, _csx, _csx, _csx, _i64, _P(_i64)]),
     ("tr_symbolic", ctypes.c_int, [_vp, _csx, _csx, _P(_i64), _P(_sz)]),
     ("tr_spgemm_flops", ctypes.c_int, [_vp, _csx, _csx, _P(_i64)]),
     ("tr_tuning_default", ctypes.c_int, [_P(tr_tuning_t)]),

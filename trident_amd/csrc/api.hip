@@ -354,6 +354,8 @@ int tr_ctx_create(tr_ctx_t *out, int device, void *hip_stream) {
         hipDeviceProp_t prop;
         TR_HIP(hipGetDeviceProperties(&prop, device));
         ctx->cu_count = prop.multiProcessorCount;
+        (void)pinned(ctx, 4096);  // staging of scalar reads, so later calls need no host allocation
+        noalloc_init(device);     // device address of the allocation-free entry points' scratch
     });
     if (rc != TR_OK) {
         delete ctx;
@@ -479,6 +481,33 @@ int tr_spadd_into(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *
         const int64_t nnz = spadd_into(ctx, A, B, out->ptr_vec, out->idx_vec, out->val, capacity);
         *nnz_needed = nnz;
         if (nnz > capacity) fail(TR_ERR_OVERFLOW, "tr_spadd_into: result needs " + std::to_string(nnz) +
+                                                      " entries, capacity " + std::to_string(capacity));
+        out->majordim = TR_ROWS;
+        out->nrows = A->nrows;
+        out->ncols = A->ncols;
+        out->nnz = nnz;
+    });
+}
+
+int tr_spadd_into_noalloc(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *out, int64_t capacity,
+                          int64_t *nnz_needed) {
+    TR_CTX_REQUIRED(ctx);
+    return guarded(ctx, [&] {
+        TR_REQUIRE(out != nullptr && nnz_needed != nullptr, "tr_spadd_into_noalloc: null output");
+        check_csx(A, "A");
+        check_csx(B, "B");
+        TR_REQUIRE(A->majordim == TR_ROWS && B->majordim == TR_ROWS, "tr_spadd_into_noalloc: operands must be CSR");
+        TR_REQUIRE(A->nrows == B->nrows && A->ncols == B->ncols, "tr_spadd_into_noalloc: shape mismatch");
+        TR_REQUIRE(capacity >= 0 && capacity <= INT32_MAX,
+                   "tr_spadd_into_noalloc: capacity must be in [0, INT32_MAX]");
+        TR_REQUIRE(out->ptr_vec != nullptr && (capacity == 0 || (out->idx_vec && out->val)),
+                   "tr_spadd_into_noalloc: output arrays missing");
+        for (const tr_csx_t *in : {A, B})
+            TR_REQUIRE(out->ptr_vec != in->ptr_vec && out->idx_vec != in->idx_vec && out->val != in->val,
+                       "tr_spadd_into_noalloc: output aliases an operand");
+        const int64_t nnz = noalloc_spadd_into(ctx, A, B, out, capacity);
+        *nnz_needed = nnz;
+        if (nnz > capacity) fail(TR_ERR_OVERFLOW, "tr_spadd_into_noalloc: result needs " + std::to_string(nnz) +
                                                       " entries, capacity " + std::to_string(capacity));
         out->majordim = TR_ROWS;
         out->nrows = A->nrows;

@@ -254,7 +254,46 @@ __global__ void k_add_rowptr(const AddTaskDev *__restrict__ tasks, int nt, int32
     if (g < nt && tasks[g].m == 0) as_global(tasks[g].optr)[0] = 0;
 }
 
+// the task table of an allocation-free add, written by a kernel from its
+// by-value argument (no host staging buffer)
+__global__ void k_put_add_task(AddTaskDev *__restrict__ dst, AddTaskDev t) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = t;
+}
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
+
+// Byte sizes of the pieces of a one-task add over `rows` rows, in the order
+// they are carved: task, nch, cstart, crow, ccount, coff, scan temp.
+struct AddFixedLayout {
+    size_t piece[7];
+    int64_t chunk_cap;
+    size_t total() const {
+        size_t t = 0;
+        for (size_t p : piece) t += align_up(p);
+        return t;
+    }
+};
+
+AddFixedLayout add_fixed_layout(int32_t rows, int64_t nnz_x, int64_t nnz_y) {
+    AddFixedLayout L;
+    const size_t R = (size_t)rows + 1;
+    L.chunk_cap = (int64_t)rows + (nnz_x + nnz_y) / kChunk + 1;
+    L.piece[0] = sizeof(AddTaskDev);
+    L.piece[1] = sizeof(int32_t) * R;
+    L.piece[2] = sizeof(int64_t) * R;
+    L.piece[3] = sizeof(int32_t) * (size_t)L.chunk_cap;
+    L.piece[4] = sizeof(int32_t) * ((size_t)L.chunk_cap + 1);
+    L.piece[5] = sizeof(int64_t) * ((size_t)L.chunk_cap + 1);
+    L.piece[6] = scan_tmp_bytes(std::max<int64_t>(L.chunk_cap, rows));
+    return L;
+}
+
 }  // namespace
+
+size_t spadd_fixed_bytes(int32_t rows, int64_t nnz_x, int64_t nnz_y) {
+    return add_fixed_layout(rows, nnz_x, nnz_y).total();
+}
 
 void add_setup(tr_ctx *ctx, Workspace &ws, AddPlan &ap, const std::vector<AddTaskDev> &tasks, int64_t nnz_x,
                int64_t nnz_y) {
@@ -323,6 +362,46 @@ int64_t spadd_into(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, int32_t *o
     t.m = A->nrows;
     t.optr = optr;
     add_setup(ctx, ws, ap, {t}, A->nnz, C->nnz);
+    add_count(ctx, ap, false);
+    const int64_t nnz = add_total(ctx, ap);
+    if (nnz > capacity) return nnz;
+    add_write(ctx, ap, oi, ov);
+    return nnz;
+}
+
+int64_t spadd_into_fixed(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, int32_t *optr, int32_t *oi, double *ov,
+                         int64_t capacity, char *base, size_t bytes) {
+    const AddFixedLayout L = add_fixed_layout(A->nrows, A->nnz, C->nnz);
+    TR_REQUIRE(L.total() <= bytes, "spadd: fixed scratch region too small");
+    char *q = base;
+    auto carve = [&](size_t b) {
+        char *p = q;
+        q += align_up(b);
+        return p;
+    };
+    AddPlan ap;
+    AddTaskDev t{};
+    t.xp = A->ptr_vec;
+    t.xi = A->idx_vec;
+    t.xv = A->val;
+    t.yp = C->ptr_vec;
+    t.yi = C->idx_vec;
+    t.yv = C->val;
+    t.m = A->nrows;
+    t.optr = optr;
+    t.row_base = 0;
+    ap.nt = 1;
+    ap.h = {t};
+    ap.rows_total = A->nrows;
+    ap.chunk_cap = L.chunk_cap;
+    ap.d = (AddTaskDev *)carve(L.piece[0]);
+    ap.nch = (int32_t *)carve(L.piece[1]);
+    ap.cstart = (int64_t *)carve(L.piece[2]);
+    ap.crow = (int32_t *)carve(L.piece[3]);
+    ap.ccount = (int32_t *)carve(L.piece[4]);
+    ap.coff = (int64_t *)carve(L.piece[5]);
+    ap.scan_tmp = (int64_t *)carve(L.piece[6]);
+    TR_LAUNCH(ctx, "put_add_task", k_put_add_task, 1, 64, 0, ap.d, t);
     add_count(ctx, ap, false);
     const int64_t nnz = add_total(ctx, ap);
     if (nnz > capacity) return nnz;

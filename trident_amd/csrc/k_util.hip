@@ -287,11 +287,17 @@ void launch_fill_i32(tr_ctx *ctx, int32_t *p, int32_t v, int64_t n) {
     TR_LAUNCH(ctx, "fill_i32", k_fill_i32, g, 256, 0, p, v, n);
 }
 
-void launch_validate(tr_ctx *ctx, const tr_csx_t *m, int32_t *d_status) {
+size_t validate_tmp_bytes(int64_t nnz) { return sizeof(uint32_t) * (size_t)(nnz / 32 + 2); }
+
+void launch_validate(tr_ctx *ctx, const tr_csx_t *m, int32_t *d_status, uint32_t *starts) {
     const int32_t P = ptrdim(m), O = otherdim(m);
-    size_t words = (size_t)(m->nnz / 32 + 2);
-    uint32_t *starts = (uint32_t *)scratch(ctx, S_MISC2, words * sizeof(uint32_t));
-    TR_HIP(hipMemsetAsync(starts, 0, words * sizeof(uint32_t), ctx->stream));
+    const size_t words = validate_tmp_bytes(m->nnz) / sizeof(uint32_t);
+    if (starts) {
+        launch_fill_i32(ctx, (int32_t *)starts, 0, (int64_t)words);
+    } else {
+        starts = (uint32_t *)scratch(ctx, S_MISC2, words * sizeof(uint32_t));
+        TR_HIP(hipMemsetAsync(starts, 0, words * sizeof(uint32_t), ctx->stream));
+    }
     TR_LAUNCH(ctx, "validate_ptr", k_validate_ptr, div_up((uint64_t)P + 1, 256), 256, 0, m->ptr_vec, P, m->nnz,
               starts, d_status);
     if (m->nnz > 0) {

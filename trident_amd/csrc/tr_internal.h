@@ -187,7 +187,10 @@ size_t scan_tmp_bytes(int64_t n);
 void launch_exclusive_scan_i64(tr_ctx *ctx, const int64_t *in, int64_t *out, int64_t n, int64_t *tmp = nullptr);
 void launch_exclusive_scan_i32_to_i64(tr_ctx *ctx, const int32_t *in, int64_t *out, int64_t n,
                                       int64_t *tmp = nullptr);
-void launch_validate(tr_ctx *ctx, const tr_csx_t *m, int32_t *d_status);
+// starts: caller-owned temp of validate_tmp_bytes(nnz) bytes, or nullptr to
+// use the context scratch.
+size_t validate_tmp_bytes(int64_t nnz);
+void launch_validate(tr_ctx *ctx, const tr_csx_t *m, int32_t *d_status, uint32_t *starts = nullptr);
 void launch_fill_i32(tr_ctx *ctx, int32_t *p, int32_t v, int64_t n);
 void launch_transpose(tr_ctx *ctx, const tr_csx_t *m, tr_csx_t *out);
 void launch_sort_rows(tr_ctx *ctx, tr_csx_t *m);
@@ -220,6 +223,19 @@ int spgemm_panels(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t max
 void spadd(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, tr_csx_t *out);
 int64_t spadd_into(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, int32_t *optr, int32_t *oi, double *ov,
                    int64_t capacity);
+// The same add with every temporary carved from the device region
+// [base, base + bytes) instead of the context's pool: no allocation.
+// spadd_fixed_bytes gives the bytes it needs (256-byte aligned pieces).
+size_t spadd_fixed_bytes(int32_t rows, int64_t nnz_x, int64_t nnz_y);
+int64_t spadd_into_fixed(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, int32_t *optr, int32_t *oi, double *ov,
+                         int64_t capacity, char *base, size_t bytes);
+
+// noalloc.hip: the per-device static scratch of the allocation-free entry
+// points; noalloc_init runs once per device (from tr_ctx_create, with the
+// device current) and noalloc_spadd_into is the body of
+// tr_spadd_into_noalloc.
+void noalloc_init(int device);
+int64_t noalloc_spadd_into(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *C, tr_csx_t *out, int64_t capacity);
 
 // Read a device value through pinned staging (synchronises the stream).
 template <typename T>
