@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_colfilter(const int32_t *__restrict__ a
                                                   const double *__restrict__ av, int32_t m,
                                                   const uint64_t *__restrict__ mask, int32_t *__restrict__ kept,
                                                   const int64_t *__restrict__ np64, int32_t *__restrict__ optr,
-                                                  int32_t *__restrict__ oi, double *__restrict__ ov) {
+                                                  int32_t *__restrict__ oi, double *__restrict__ ov, int64_t cap) {
     const int l = lane_id();
     const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     int32_t a0 = 0, a1 = 0;
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_colfilter(const int32_t *__restrict__ a
         for (int32_t p = a0; p < a1; ++p) {
             const int32_t col = ai[p];
             if ((mask[col >> 6] >> (col & 63)) & 1ull) {
-                if (kWrite) {
+                if (kWrite && base + c < cap) {
                     oi[base + c] = col;
                     ov[base + c] = av[p];
                 }
@@ -103,14 +103,72 @@ __global__ __launch_bounds__(256) void k_colfilter(const int32_t *__restrict__ a
             const uint64_t mk = ballot(keep);
             if (kWrite && keep) {
                 const int64_t q = sb + run + rank_below(mk);
-                oi[q] = col;
-                ov[q] = av[p];
+                if (q < cap) {
+                    oi[q] = col;
+                    ov[q] = av[p];
+                }
             }
             run += __popcll(mk);
         }
         if (l == src) c = run;
     }
     if (!kWrite && r < m) kept[r] = c;
+}
+
+// ---- replay guard ----
+// A replay reuses every size of the plan: work-list capacities, the product
+// and output arenas and each task's place in them.  It is correct only for
+// operands with the planned sparsity pattern (values may change).  The
+// guards compare what a replay recomputes with the plan before each phase
+// that writes into planned storage; on a mismatch they record it in the
+// guard word (tr_batch_check reports it) and zero the device-side counts the
+// later kernels loop over, so nothing is written outside the batch's
+// buffers.  Bits: 8 operand nnz changed, 1 products differ from the plan,
+// 2/4 (the add) chunk or output offsets differ.
+struct GuardOperand {
+    const int32_t *ptr;
+    int32_t n;      // ptr has n + 1 entries
+    int32_t pad;
+    int64_t nnz;    // planned ptr[n]
+};
+
+__global__ void k_guard_reset(int32_t *__restrict__ guard) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *guard = 0;
+}
+
+__global__ void k_guard_operands(const GuardOperand *__restrict__ ops, int nops, int32_t *__restrict__ guard) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nops) {
+        const GuardOperand o = ops[i];
+        if (o.ptr[0] != 0 || (int64_t)o.ptr[o.n] != o.nnz) atomicOr(guard, 8);
+    }
+}
+
+// after the analysis: a flagged replay runs no symbolic or numeric rows
+__global__ void k_guard_counts(Counters *__restrict__ cnt, const int32_t *__restrict__ guard) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && *guard) {
+        for (int c = 0; c < kNumClasses; ++c) cnt->cls[c] = 0;
+        cnt->items = 0;
+    }
+}
+
+// after the symbolic phase: every task's products start where the plan put
+// them (expect[t], expect[nt] = total), the heavy work lists fit and the
+// cursor total is the planned one; otherwise no numeric row runs
+__global__ void k_guard_products(const TaskDev *__restrict__ tasks, int nt, const int64_t *__restrict__ rowptr,
+                                 int32_t rows, const int64_t *__restrict__ expect, Counters *__restrict__ cnt,
+                                 const int64_t *__restrict__ cbase, int32_t item_cap, int64_t cursor_cap,
+                                 int32_t *__restrict__ guard) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int bad = cnt->overflow & 1;
+    for (int t = 0; t <= nt && !bad; ++t)
+        if (rowptr[t < nt ? tasks[t].row_base : rows] != expect[t]) bad = 1;
+    if (cbase && cbase[item_cap] > cursor_cap) bad = 1;
+    if (bad) atomicOr(guard, 1);
+    if (*guard) {
+        for (int c = 0; c < kNumClasses; ++c) cnt->cls[c] = 0;
+        cnt->items = 0;
+    }
 }
 
 __global__ void k_zero_i32(int32_t *__restrict__ p, int64_t n) {
@@ -163,6 +221,12 @@ struct tr_batch {
     std::vector<int64_t> optr_off;   // per task: first row pointer in o_ptr
     std::vector<int64_t> o_nnz;
     int64_t flops = 0;
+    // replay guard: flag word, planned operand ends, planned per-task
+    // product and output offsets (nt + 1 each)
+    int32_t *guard = nullptr;
+    GuardOperand *guard_ops = nullptr;
+    int guard_nops = 0;
+    int64_t *expect_p = nullptr, *expect_o = nullptr;
     bool no_accum = false;  // no task has an accumulator: results are the products
     bool single_run = false;  // workspace in the context's scratch slots: no replay
     hipGraph_t graph = nullptr;
@@ -209,11 +273,11 @@ void batch_compress(tr_batch *b) {
         BatchOperand &ac = b->Ac[t];
         TR_LAUNCH(ctx, "colfilter_count", k_colfilter<false>, div_up((uint64_t)std::max(A.nrows, 1), 256), 256, 0,
                   A.ptr_vec, A.idx_vec, A.val, A.nrows, mask, b->kept_tmp[t], (const int64_t *)nullptr,
-                  (int32_t *)nullptr, (int32_t *)nullptr, (double *)nullptr);
+                  (int32_t *)nullptr, (int32_t *)nullptr, (double *)nullptr, (int64_t)0);
         launch_exclusive_scan_i32_to_i64(ctx, b->kept_tmp[t], b->np64[t], A.nrows, b->scan_tmp[t]);
         TR_LAUNCH(ctx, "colfilter_write", k_colfilter<true>, div_up((uint64_t)std::max(A.nrows, 1), 256), 256, 0,
                   A.ptr_vec, A.idx_vec, A.val, A.nrows, mask, (int32_t *)nullptr, b->np64[t], ac.optr, ac.oidx,
-                  ac.oval);
+                  ac.oval, A.nnz);
         if (A.nrows == 0) TR_LAUNCH(ctx, "zero_ptr", k_zero_i32, 1, 64, 0, ac.optr, (int64_t)1);
         // B: row segments
         BatchOperand &bc = b->Bc[t];
@@ -226,11 +290,22 @@ void batch_compress(tr_batch *b) {
 // enqueue multiply + add; capture == true: sizes from the eager run
 void batch_body(tr_batch *b, bool capture) {
     tr_ctx *ctx = b->ctx;
+    const bool guarded = capture && b->guard;  // replays check the plan
+    if (guarded) {
+        TR_LAUNCH(ctx, "batch_guard", k_guard_reset, 1, 64, 0, b->guard);
+        TR_LAUNCH(ctx, "batch_guard", k_guard_operands, div_up((uint64_t)b->guard_nops, 64), 64, 0, b->guard_ops,
+                  b->guard_nops, b->guard);
+    }
     if (b->mode) batch_compress(b);
     mul_analyze(ctx, b->mp);
+    if (guarded) TR_LAUNCH(ctx, "batch_guard", k_guard_counts, 1, 64, 0, b->mp.cnt, b->guard);
     if (!capture) mul_read_counts(ctx, b->mp);
     mul_symbolic(ctx, *b->ws_mul, b->mp, true, capture);
     const int64_t ncur = mul_cursor_prep(ctx, *b->ws_mul, b->mp, capture);
+    if (guarded)
+        TR_LAUNCH(ctx, "batch_guard", k_guard_products, 1, 64, 0, b->mp.d_tasks, b->nt, b->mp.rowptr,
+                  b->mp.rows_total, b->expect_p, b->mp.cnt, b->mp.h.items > 0 ? b->mp.cbase : nullptr,
+                  b->mp.item_cap, b->mp.cursor_cap, b->guard);
     if (!capture) {
         b->p_total = read_scalar<int64_t>(ctx, b->mp.rowptr + b->mp.rows_total);
         mul_read_counts(ctx, b->mp);
@@ -461,6 +536,39 @@ int tr::batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_c
         for (int t = 0; t < ntasks; ++t)
             if (b->o_nnz[t] > INT32_MAX) fail(TR_ERR_OVERFLOW, "batch: an accumulated tile exceeds INT32_MAX entries");
         b->ran = true;  // the eager pass already produced the results
+        if (!b->single_run) {
+            // what a replay must reproduce: operand ends and task offsets
+            std::vector<GuardOperand> go;
+            auto add_op = [&](const tr_csx_t &m) {
+                if (m.ptr_vec) go.push_back(GuardOperand{m.ptr_vec, m.nrows, 0, m.nnz});
+            };
+            for (int t = 0; t < ntasks; ++t) {
+                add_op(b->A[t]);
+                add_op(b->B[t]);
+                add_op(b->Cin[t]);
+            }
+            std::vector<int64_t> ep(ntasks + 1), eo(ntasks + 1);
+            for (int t = 0; t <= ntasks; ++t) {
+                const int32_t g = t < ntasks ? b->mp.h_tasks[t].row_base : b->mp.rows_total;
+                ep[t] = read_scalar<int64_t>(ctx, b->mp.rowptr + g);
+                eo[t] = t < ntasks ? b->o_base[t] : b->o_total;
+            }
+            b->guard_nops = (int)go.size();
+            b->guard = (int32_t *)b->own(sizeof(int32_t));
+            b->guard_ops = (GuardOperand *)b->own(sizeof(GuardOperand) * go.size());
+            b->expect_p = (int64_t *)b->own(sizeof(int64_t) * ep.size());
+            b->expect_o = (int64_t *)b->own(sizeof(int64_t) * eo.size());
+            TR_HIP(hipMemcpyAsync(b->guard_ops, go.data(), sizeof(GuardOperand) * go.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+            TR_HIP(hipMemcpyAsync(b->expect_p, ep.data(), sizeof(int64_t) * ep.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+            TR_HIP(hipMemcpyAsync(b->expect_o, eo.data(), sizeof(int64_t) * eo.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+            TR_LAUNCH(ctx, "batch_guard", k_guard_reset, 1, 64, 0, b->guard);
+            b->ap.guard = b->guard;
+            b->ap.expect = b->expect_o;
+            TR_HIP(hipStreamSynchronize(ctx->stream));  // host staging of the copies above ends here
+        }
         if (flags & TR_BATCH_NO_GRAPH) {
             ctx->timing = timing;
             return;
@@ -506,21 +614,51 @@ int tr_batch_run(tr_batch_t b) {
     });
 }
 
+// A view of task t's result.  Reads only fields fixed when the batch was
+// created and never touches the context (its last-error string belongs to
+// the thread that owns the context), so any thread may call it while the
+// owner keeps using the context.
 int tr_batch_result(tr_batch_t b, int32_t t, tr_csx_t *C_out) {
+    if (!b || !C_out || t < 0 || t >= b->nt) return TR_ERR_INVALID;
+    tr_csx_t c{};
+    c.majordim = TR_ROWS;
+    c.nrows = b->A[t].nrows;
+    c.ncols = b->B[t].ncols;
+    c.nnz = b->o_nnz[t];
+    c.ptr_vec = b->o_ptr + b->optr_off[t];
+    c.idx_vec = b->o_idx + b->o_base[t];
+    c.val = b->o_val + b->o_base[t];
+    c.contig = 0;
+    *C_out = c;
+    return TR_OK;
+}
+
+}  // extern "C"
+
+// Device bytes the batch holds between runs (its own blocks: operands after
+// compression, product and output arenas, row pointers), without the
+// context scratch slots a single-run batch borrows while it runs.
+size_t tr::batch_retained_bytes(tr_batch_t b) {
+    size_t t = 0;
+    for (void *p : b->owned) t += b->ctx->pool.size_of(p);
+    if (b->ws_mul && b->ws_mul->owned)
+        for (size_t i = 0; i < b->ws_mul->p.size(); ++i) t += b->ws_mul->p[i] ? b->ws_mul->sz[i] : 0;
+    if (b->ws_add && b->ws_add->owned)
+        for (size_t i = 0; i < b->ws_add->p.size(); ++i) t += b->ws_add->p[i] ? b->ws_add->sz[i] : 0;
+    return t;
+}
+
+extern "C" {
+
+int tr_batch_check(tr_batch_t b) {
     if (!b) return TR_ERR_INVALID;
     tr_ctx *ctx = b->ctx;
     return bguard(ctx, [&] {
-        TR_REQUIRE(t >= 0 && t < b->nt && C_out, "tr_batch_result: bad task index");
-        tr_csx_t c{};
-        c.majordim = TR_ROWS;
-        c.nrows = b->A[t].nrows;
-        c.ncols = b->B[t].ncols;
-        c.nnz = b->o_nnz[t];
-        c.ptr_vec = b->o_ptr + b->optr_off[t];
-        c.idx_vec = b->o_idx + b->o_base[t];
-        c.val = b->o_val + b->o_base[t];
-        c.contig = 0;
-        *C_out = c;
+        if (!b->guard) return;  // single-run batches are never replayed
+        const int32_t g = read_scalar<int32_t>(ctx, b->guard);
+        if (g)
+            fail(TR_ERR_INVALID, "tr_batch_check: the last run found operands whose sparsity pattern differs from "
+                                 "the planned one (guard bits " + std::to_string(g) + "); its results are not valid");
     });
 }
 

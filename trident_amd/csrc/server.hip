@@ -52,14 +52,17 @@ struct Request {
     int status = TR_OK;
     std::string error;
     size_t bound = 0;            // admitted bytes of the current step
+    int oom_retries = 0;         // times the step was put back after a failed allocation
     std::shared_ptr<Batch> partial;  // AP of a Galerkin request
     int32_t partial_task = 0;
     std::shared_ptr<Batch> result;   // final product
     int32_t result_task = 0;
 };
 
-// A finished batch; stays alive while a result or partial refers to it.  Its
-// admitted memory bound is returned to the budget when it is destroyed.
+// A finished batch; stays alive while a result or partial refers to it.
+// While it runs it counts its admitted bound against the budget; once it has
+// finished, only the device bytes it keeps (its results) stay counted, and
+// they are returned to the budget when it is destroyed.
 // Each worker's pool reserves the worker's share of the budget once and
 // places every batch's blocks in it (tr::Arena): a hipMalloc larger than
 // what the runtime has cached from earlier frees takes about 14 ms per GB
@@ -107,6 +110,7 @@ struct tr_server {
     std::vector<Operand> ops;
     int64_t next_id = 1;
     std::atomic<size_t> in_use{0};
+    int running = 0;  // batches admitted and not yet finished (s->mu)
     bool stopping = false;
     // symbolic memory bound per (kind, operand a, operand b, step): operands
     // are resident and products deterministic, so a bound is computed once
@@ -176,7 +180,15 @@ void worker_loop(tr_server *s, Worker *w) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv.wait(lk, [&] { return s->stopping || !s->queue.empty(); });
-            if (s->stopping && s->queue.empty()) return;
+            if (s->stopping) {
+                // requests still waiting will not run: fail them so waiters return
+                while (!s->queue.empty()) {
+                    Request *q = s->queue.front();
+                    s->queue.pop_front();
+                    finish_step(s, q, nullptr, 0, TR_ERR_INVALID, "server stopped before the request ran");
+                }
+                return;
+            }
             // admission in arrival order while the symbolic bound fits; a step
             // whose bound is not known yet is taken out, its bound computed
             // outside the server lock, and put back at the front
@@ -214,7 +226,15 @@ void worker_loop(tr_server *s, Worker *w) {
                     finish_step(s, r, nullptr, 0, TR_ERR_OOM, "request exceeds the serving memory budget");
                     continue;
                 }
-                if (s->in_use.load() + admitted + bytes > s->budget) break;  // wait for memory to free up
+                if (s->in_use.load() + admitted + bytes > s->budget) {
+                    // A Galerkin continuation is the only way to free its own
+                    // AP: with no batch running on any worker, nothing else can
+                    // return memory, so it runs now (an allocation that then
+                    // fails ends it with TR_ERR_OOM instead of waiting forever).
+                    // Anything else waits for memory to free up.
+                    const bool continuation = r->kind == TR_REQ_GALERKIN && r->step == 1;
+                    if (!(continuation && take.empty() && s->running == 0)) break;
+                }
                 tr_csx_t A, B;
                 step_operands(s, r, A, B);
                 s->queue.pop_front();
@@ -230,6 +250,7 @@ void worker_loop(tr_server *s, Worker *w) {
                 continue;
             }
             s->in_use.fetch_add(admitted);
+            ++s->running;
         }
         // run the batch (flops of first steps counted for the request)
         auto bt = std::make_shared<Batch>();
@@ -290,11 +311,42 @@ void worker_loop(tr_server *s, Worker *w) {
                 w->ctx->pool.n_malloc - nm0, w->ctx->pool.n_free - nf0, rc, mem_free / 1e9,
                 w->ctx->pool.in_use() / 1e9, w->ctx->pool.cached() / 1e9);
 #endif
+        if (rc == TR_OK) {
+            // from now on the batch holds only its results
+            size_t keep;
+            {
+                std::lock_guard<std::mutex> ck(w->mu);
+                keep = tr::batch_retained_bytes(bt->b);
+            }
+            if (keep < bt->bound) {
+                s->in_use.fetch_sub(bt->bound - keep);
+                bt->bound = keep;
+            }
+        }
         std::lock_guard<std::mutex> lk(s->mu);
+        --s->running;
         ++s->batches;
         s->flops_done += fl;
+        if (rc == TR_ERR_OOM && !s->stopping) {
+            // put the steps back (in their order, ahead of newer arrivals) to be
+            // admitted again, in smaller batches as memory allows; a step that
+            // already failed to allocate once fails for good
+            for (size_t t = take.size(); t-- > 0;) {
+                Request *r = take[t];
+                if (r->oom_retries++ == 0 && take.size() > 1) {
+                    r->bound = 0;
+                    s->queue.push_front(r);
+                } else {
+                    finish_step(s, r, nullptr, 0, rc, err);
+                }
+            }
+            s->cv.notify_all();
+            s->done_cv.notify_all();
+            continue;
+        }
         for (size_t t = 0; t < take.size(); ++t)
             finish_step(s, take[t], rc == TR_OK ? bt : nullptr, (int32_t)t, rc, err);
+        s->done_cv.notify_all();
     }
 }
 

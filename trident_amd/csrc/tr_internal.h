@@ -143,6 +143,10 @@ enum ScratchSlot : int {
 };
 
 void *scratch(tr_ctx *ctx, int slot, size_t bytes);
+// Bytes the pool sets aside for a request of `b` bytes (256-byte steps up to
+// 1 MiB, 2 MiB granules above); scratch slots grow to 1.25x a request.
+size_t pool_block_bytes(size_t b);
+inline size_t scratch_block_bytes(size_t b) { return pool_block_bytes(b + b / 4); }
 void *pinned(tr_ctx *ctx, size_t bytes);
 
 // Launch bracketing for per-kernel timing.
@@ -212,6 +216,7 @@ void check_operand_shapes(const tr_csx_t *A, const tr_csx_t *B);  // host-side s
 constexpr int32_t kBatchSingleRun = 1 << 30;
 int batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_csx_t *B, const tr_csx_t *C_in,
                  int32_t flags, bool validate, tr_batch_t *out);
+size_t batch_retained_bytes(tr_batch_t b);  // device bytes a finished batch keeps
 int64_t spgemm_flops(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B);
 // Runs the symbolic phase and returns nnz(C) (and bytes needed).
 void spgemm_symbolic_only(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *nnz, size_t *bytes);
