@@ -142,10 +142,12 @@ int tr_spadd_into(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *
 int tr_spadd_into_noalloc(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *out, int64_t capacity,
                           int64_t *nnz_needed);
 
-/* Exact nnz(C) and the device bytes tr_spgemm needs for C = A*B (output tile
- * plus peak scratch): the admission bound used by the serving loop.  No
- * counterpart in the reference (its buffers grow on demand,
- * include/utils.cuh:671-716). */
+/* Exact nnz(C) and an upper bound of the device bytes tr_spgemm takes for
+ * C = A*B from a context pool with nothing cached (every scratch array and
+ * work list at the size the product requests, the output tile, and the
+ * product arena of a batch): the admission bound used by the serving loop.
+ * Runs the product's symbolic steps.  No counterpart in the reference (its
+ * buffers grow on demand, include/utils.cuh:671-716). */
 int tr_symbolic(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t *nnz_C,
                 size_t *bytes_needed);
 
@@ -199,10 +201,15 @@ int tr_ctx_get_tuning(tr_ctx_t ctx, tr_tuning_t *t);
  *    kernel by kernel, which also lets per-kernel timing see it.
  * tr_batch_create plans the stage (one eager pass that sizes every buffer
  * and already produces the results) and captures it; tr_batch_run replays
- * the graph on the context stream with no host round trip.  Inputs must keep
- * their shapes and addresses between create and run.  Results are
- * batch-owned (tr_batch_result) and stay valid until the next run or
- * tr_batch_destroy. */
+ * the graph on the context stream with no host round trip.  Between create
+ * and run the operands must keep their addresses, shapes and sparsity
+ * patterns (ptr_vec and idx_vec contents): only values may change, since a
+ * replay reuses every size of the plan.  Each replay checks this on the
+ * device before it writes into planned storage; on a mismatch it writes
+ * nothing out of bounds and tr_batch_check (which synchronises the stream)
+ * returns TR_ERR_INVALID for that run.  Results are batch-owned
+ * (tr_batch_result, callable from any thread) and stay valid until the next
+ * run or tr_batch_destroy. */
 #define TR_BATCH_COMPRESS 1
 #define TR_BATCH_NO_GRAPH 2
 typedef struct tr_batch *tr_batch_t;
@@ -210,6 +217,7 @@ int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_cs
                     int32_t flags, tr_batch_t *batch);
 int tr_batch_run(tr_batch_t batch);
 int tr_batch_result(tr_batch_t batch, int32_t task, tr_csx_t *C_out);
+int tr_batch_check(tr_batch_t batch);
 int tr_batch_stats(tr_batch_t batch, int64_t *flops, int64_t *nnz_products, int64_t *nnz_out);
 int tr_batch_destroy(tr_batch_t batch);
 

@@ -1,12 +1,14 @@
 """GPU parity of the batched stage executor (tr_batch_*): one SUMMA stage of
 a 4x4 tile grid, C_ij <- C_ij + A_ik * A_kj for all 16 (i, j), as one
 hipGraph launch, against the CPU oracle — bit-exact, with and without
-bitmask compression, across repeated graph replays."""
+bitmask compression, across repeated graph replays; a replay with a changed
+sparsity pattern is caught by the device-side guard."""
 import numpy as np
 import pytest
 
 import oracle
 from trident_amd import gen
+from trident_amd._lib import TR_ERR_INVALID, TridentError
 from trident_amd.device import Batch, sp_mma, DeviceCsx
 
 pytestmark = pytest.mark.gpu
@@ -113,3 +115,52 @@ def test_eager_batch_replays_match_graph(ctx, compress):
             assert same(be.result(t).download(), bg.result(t).download()), f"run {rep} task {keys[t]}"
     bg.close()
     be.close()
+
+
+def _narrow_and_spread(n, per_row, seed):
+    """Two CSR matrices with the same row pointers (so the same nnz): one whose
+    rows all use columns 0..per_row-1, one whose rows use random columns."""
+    rng = np.random.default_rng(seed)
+    ptr = np.arange(0, (n + 1) * per_row, per_row, dtype=np.int64)
+    narrow = np.tile(np.arange(per_row, dtype=np.int32), n)
+    spread = np.concatenate([np.sort(rng.choice(n, per_row, replace=False)) for _ in range(n)]).astype(np.int32)
+    val = rng.uniform(-1, 1, n * per_row)
+    return gen.from_arrays(n, n, ptr, narrow, val), gen.from_arrays(n, n, ptr, spread, val)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+@pytest.mark.parametrize("accum", [False, True])
+def test_replay_guard_rejects_changed_pattern(ctx, graph, accum):
+    """A replay reuses the plan's sizes, so only values may change between
+    create and run.  Changed values: the replay is checked and exact.  A
+    changed pattern (same nnz, columns spread out so the product grows past
+    the planned arena): the run writes nothing out of bounds and
+    tr_batch_check reports it; restoring the pattern makes runs valid again."""
+    n = 512
+    A = gen.erdos_renyi(n, 8, seed=5)
+    Bn, Bs = _narrow_and_spread(n, 4, seed=6)
+    C0 = prod(A, gen.erdos_renyi(n, 4, seed=7)) if accum else None
+    print(f"replay guard: graph={graph} accum={accum}", flush=True)
+    dA, dB = ctx.upload(A), ctx.upload(Bn)
+    dC = [ctx.upload(C0)] if accum else None
+    b = Batch(ctx, [dA], [dB], dC, graph=graph)
+    b.check()
+    assert same(b.result(0).download(), ref_add(prod(A, Bn), C0))
+    # values only: B's values change, its pattern does not
+    Bv = gen.from_arrays(n, n, Bn.ptr, Bn.idx, -2.0 * Bn.val)
+    ctx.check(ctx.lib.tr_memcpy_h2d(ctx.h, dB.d.val, Bv.val.ctypes.data, Bv.val.nbytes))
+    b.run()
+    b.check()
+    assert same(b.result(0).download(), ref_add(prod(A, Bv), C0))
+    # pattern change: the planned product arena is too small for it
+    ctx.check(ctx.lib.tr_memcpy_h2d(ctx.h, dB.d.idx_vec, Bs.idx.ctypes.data, Bs.idx.nbytes))
+    b.run()
+    with pytest.raises(TridentError) as e:
+        b.check()
+    assert e.value.code == TR_ERR_INVALID
+    # back to the planned pattern: valid again
+    ctx.check(ctx.lib.tr_memcpy_h2d(ctx.h, dB.d.idx_vec, Bn.idx.ctypes.data, Bn.idx.nbytes))
+    b.run()
+    b.check()
+    assert same(b.result(0).download(), ref_add(prod(A, Bv), C0))
+    b.close()

@@ -71,10 +71,38 @@ __global__ void k_add_chunks(const AddTaskDev *__restrict__ tasks, int nt, int32
     nch[g] = (int32_t)((len + kChunk - 1) / kChunk);
 }
 
-__global__ void k_add_chunk_rows(const int64_t *__restrict__ cstart, int32_t rows, int32_t *__restrict__ crow) {
+__global__ void k_add_chunk_rows(const int64_t *__restrict__ cstart, int32_t rows, int32_t *__restrict__ crow,
+                                 int64_t cap) {
     const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= rows) return;
-    for (int64_t c = cstart[g]; c < cstart[g + 1]; ++c) crow[c] = g;
+    const int64_t c1 = min(cstart[g + 1], cap);  // cap: the planned chunk capacity (replays)
+    for (int64_t c = cstart[g]; c < c1; ++c) crow[c] = g;
+}
+
+// Replay guards of a planned add (a batch's graph): the chunk count must fit
+// the planned chunk capacity and every task's output must start where the
+// plan put it.  On a mismatch the guard word records it and the device-side
+// chunk count (cstart[rows]) is zeroed, so the add passes write nothing.
+__global__ void k_add_guard_chunks(int64_t *__restrict__ cstart, int32_t rows, int64_t cap,
+                                   int32_t *__restrict__ guard) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (cstart[rows] > cap) atomicOr(guard, 2);
+    if (*guard) cstart[rows] = 0;
+}
+
+__global__ void k_add_guard_output(const AddTaskDev *__restrict__ tasks, int nt, int64_t *__restrict__ cstart,
+                                   const int64_t *__restrict__ coff, int32_t rows, int64_t nchunks,
+                                   const int64_t *__restrict__ expect, int32_t *__restrict__ guard) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t <= nt && *guard == 0) {
+        // expect[t]: planned first output entry of task t (expect[nt]: total)
+        const int64_t got = t < nt ? coff[cstart[tasks[t].row_base]] : coff[nchunks];
+        if (got != expect[t]) atomicOr(guard, 4);
+    }
+}
+
+__global__ void k_add_guard_apply(int64_t *__restrict__ cstart, int32_t rows, const int32_t *__restrict__ guard) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && *guard) cstart[rows] = 0;
 }
 
 // merge-path split of diagonal t: number of X entries among the first t
@@ -324,7 +352,10 @@ void add_count(tr_ctx *ctx, AddPlan &ap, bool capture) {
     const int32_t rows = ap.rows_total;
     TR_LAUNCH(ctx, "add_chunks", k_add_chunks, div_up((uint64_t)rows, 256), 256, 0, ap.d, ap.nt, rows, ap.nch);
     launch_exclusive_scan_i32_to_i64(ctx, ap.nch, ap.cstart, rows, ap.scan_tmp);
-    TR_LAUNCH(ctx, "add_chunk_rows", k_add_chunk_rows, div_up((uint64_t)rows, 256), 256, 0, ap.cstart, rows, ap.crow);
+    TR_LAUNCH(ctx, "add_chunk_rows", k_add_chunk_rows, div_up((uint64_t)rows, 256), 256, 0, ap.cstart, rows, ap.crow,
+              ap.chunk_cap);
+    if (capture && ap.guard)
+        TR_LAUNCH(ctx, "add_guard", k_add_guard_chunks, 1, 64, 0, ap.cstart, rows, ap.chunk_cap, ap.guard);
     if (!capture) {
         ap.nchunks = read_scalar<int64_t>(ctx, ap.cstart + rows);
         TR_REQUIRE(ap.nchunks <= ap.chunk_cap, "spadd: chunk bound violated");
@@ -333,6 +364,11 @@ void add_count(tr_ctx *ctx, AddPlan &ap, bool capture) {
               ap.d, ap.nt, rows, ap.cstart, ap.crow, ap.ccount, (const int64_t *)nullptr, (int32_t *)nullptr,
               (double *)nullptr);
     launch_exclusive_scan_i32_to_i64(ctx, ap.ccount, ap.coff, ap.nchunks, ap.scan_tmp);
+    if (capture && ap.guard && ap.expect) {
+        TR_LAUNCH(ctx, "add_guard", k_add_guard_output, div_up((uint64_t)ap.nt + 1, 64), 64, 0, ap.d, ap.nt,
+                  ap.cstart, ap.coff, rows, ap.nchunks, ap.expect, ap.guard);
+        TR_LAUNCH(ctx, "add_guard", k_add_guard_apply, 1, 64, 0, ap.cstart, rows, ap.guard);
+    }
 }
 
 int64_t add_total(tr_ctx *ctx, AddPlan &ap) { return read_scalar<int64_t>(ctx, ap.coff + ap.nchunks); }

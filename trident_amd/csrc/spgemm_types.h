@@ -85,7 +85,9 @@ struct Workspace {
     bool owned = false;
     std::vector<void *> p;
     std::vector<size_t> sz;
-    Workspace(tr_ctx *c, bool own) : ctx(c), owned(own), p(S_NUM_SLOTS, nullptr), sz(S_NUM_SLOTS, 0) {}
+    std::vector<size_t> req;  // largest request per slot (sizes the memory bound of a call)
+    Workspace(tr_ctx *c, bool own)
+        : ctx(c), owned(own), p(S_NUM_SLOTS, nullptr), sz(S_NUM_SLOTS, 0), req(S_NUM_SLOTS, 0) {}
     void *get(int slot, size_t bytes);
     void release();
     ~Workspace() { release(); }
@@ -141,6 +143,10 @@ struct AddPlan {
     int64_t *cstart = nullptr, *coff = nullptr;
     int64_t chunk_cap = 0, nchunks = 0;
     int64_t *scan_tmp = nullptr;
+    // replay guard (batches): device word set on a mismatch with the plan,
+    // and the planned first output entry of every task (nt + 1 entries)
+    int32_t *guard = nullptr;
+    const int64_t *expect = nullptr;
 };
 
 void add_setup(tr_ctx *ctx, Workspace &ws, AddPlan &ap, const std::vector<AddTaskDev> &tasks, int64_t nnz_x,

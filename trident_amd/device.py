@@ -371,6 +371,10 @@ class Batch:
     def run(self):
         self.ctx.check(self.ctx.lib.tr_batch_run(self.h), "tr_batch_run")
 
+    def check(self):
+        """Raises if the last run met operands whose pattern differs from the plan."""
+        self.ctx.check(self.ctx.lib.tr_batch_check(self.h), "tr_batch_check")
+
     def result(self, t: int) -> DeviceCsx:
         d = tr_csx_t()
         self.ctx.check(self.ctx.lib.tr_batch_result(self.h, t, ctypes.byref(d)), "tr_batch_result")
