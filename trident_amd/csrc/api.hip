@@ -13,7 +13,7 @@ namespace tr {
 [[noreturn]] void fail(int code, const std::string &msg) { throw Error{code, msg}; }
 
 // ------------------------------------------------------------------ pool
-static size_t round_bytes(size_t b) {
+size_t pool_block_bytes(size_t b) {
     if (b == 0) b = 1;
     if (b <= (1u << 20)) return (b + 255) & ~(size_t)255;
     return (b + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);  // 2 MiB granules
@@ -28,7 +28,7 @@ Pool::~Pool() {
 
 bool Pool::reserve(size_t bytes) {
     if (base_ || bytes == 0) return base_ != nullptr;
-    const size_t b = round_bytes(bytes);
+    const size_t b = pool_block_bytes(bytes);
     if (hipMalloc(&base_, b) != hipSuccess) {
         (void)hipGetLastError();
         base_ = nullptr;
@@ -40,7 +40,9 @@ bool Pool::reserve(size_t bytes) {
 }
 
 void *Pool::alloc(size_t bytes) {
-    size_t b = round_bytes(bytes);
+    size_t b = pool_block_bytes(bytes);
+    // the test limit applies to every block handed out, cached ones included
+    if (limit_ && in_use_ + b > limit_) fail(TR_ERR_OOM, "device pool limit reached");
     if (base_) {
         size_t off = 0;
         if (arena_.alloc(b, off)) {
@@ -55,7 +57,7 @@ void *Pool::alloc(size_t bytes) {
         }
     }
     auto it = free_.lower_bound(b);
-    if (it != free_.end() && it->first <= 2 * b + (4u << 20)) {
+    if (it != free_.end() && it->first <= 2 * b + (4u << 20) && !(limit_ && in_use_ + it->first > limit_)) {
         void *p = it->second;
         size_t sz = it->first;
         free_.erase(it);

@@ -1344,6 +1344,7 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
 // ======================================================================= host side
 
 void *Workspace::get(int slot, size_t bytes) {
+    req[slot] = std::max(req[slot], bytes);
     if (!owned) return scratch(ctx, slot, bytes);
     if (bytes == 0) bytes = 1;
     if (sz[slot] < bytes) {
@@ -1596,16 +1597,30 @@ void spgemm_symbolic_only(tr_ctx *ctx, const tr_csx_t *A, const tr_csx_t *B, int
     mul_setup(ctx, ws, pl, {task_of_pair(A, B)});
     mul_analyze(ctx, pl);
     mul_read_counts(ctx, pl);
-    mul_symbolic(ctx, ws, pl, false, false);
+    // the same steps as the product up to its numeric phase, so every work
+    // list and the cursor array are sized exactly as the product sizes them
+    mul_symbolic(ctx, ws, pl, true, false);
+    const int64_t ncur = mul_cursor_prep(ctx, ws, pl, false);
     const int64_t total = read_scalar<int64_t>(ctx, pl.rowptr + pl.rows_total);
     if (nnz) *nnz = total;
     if (bytes) {
-        // output tile + work lists + cursors bound + row arrays
-        const size_t out = (size_t)total * 12 + ((size_t)pl.rows_total + 1) * 4;
-        const size_t heavy = (size_t)(pl.h.heavy_flops / std::min<int64_t>(pl.tune.window_entries, kSparseH) + 16) *
-                             (sizeof(Window) + sizeof(Item));
-        const size_t rows = ((size_t)pl.rows_total + 1) * (8 * 3 + 4 * kNumClasses);
-        *bytes = out + heavy + rows + (size_t)A->nnz * 4;
+        // Upper bound of the device memory tr_spgemm takes from a context
+        // pool with nothing cached: every scratch slot at the size the product
+        // requests (grown to 1.25x, in pool granules), the cursor array, the
+        // operand checks and the output tile.  A batch places the product in
+        // two blocks (indices, values) and its row pointers in a third.
+        std::vector<size_t> req = ws.req;
+        if (pl.h.items > 0) req[S_CURSORS] = std::max(req[S_CURSORS], sizeof(int4) * (size_t)std::max<int64_t>(ncur, 1));
+        req[S_MISC2] = std::max(req[S_MISC2], validate_tmp_bytes(std::max(A->nnz, B->nnz)));
+        req[S_TMP_CSX] = std::max(req[S_TMP_CSX], std::max<size_t>(64, sizeof(Panel)));
+        size_t b = 0;
+        for (size_t r : req)
+            if (r) b += scratch_block_bytes(r);
+        const size_t rows1 = (size_t)pl.rows_total + 1;
+        b += pool_block_bytes((size_t)total * 12 + rows1 * 4);
+        b += pool_block_bytes((size_t)total * 4) + pool_block_bytes((size_t)total * 8) + pool_block_bytes(rows1 * 4) +
+             2 * pool_block_bytes(sizeof(Panel) + rows1 * 4);
+        *bytes = b;
     }
 }
 
