@@ -58,24 +58,39 @@ def write_bmtx(path: str, m: Csr, index_bytes: int = 4, value_bytes: int = 4) ->
         f.write(arr.tobytes())
 
 
+def wire_header(nnz: int, buf_size: int, index_dtype=np.int32) -> bytes:
+    """The tile's size message {nnz, buf_size}: two integers of the index type,
+    as the reference sends them (`IT payload[2]`, include/tile_holder.cuh:121).
+    Raises OverflowError when either value does not fit the index type (an
+    int32 tile's buffer is limited to 2^31 - 1 bytes)."""
+    info = np.iinfo(index_dtype)
+    for name, v in (("nnz", nnz), ("buf_size", buf_size)):
+        if v < 0 or v > info.max:
+            raise OverflowError(f"wire tile: {name} = {v} does not fit {np.dtype(index_dtype).name}")
+    return np.array([nnz, buf_size], dtype=index_dtype).tobytes()
+
+
 def tile_to_wire(m: Csr, index_dtype=np.int32, value_dtype=np.float64) -> bytes:
-    """Serialise a tile as header {nnz, buf_size} + [val | idx | ptr]."""
+    """Serialise a tile as header {nnz, buf_size} + [val | idx | ptr]
+    (include/tile_holder.cuh:80-89 buffer order, :121 header)."""
     val = np.ascontiguousarray(m.val, value_dtype).tobytes()
     idx = np.ascontiguousarray(m.idx, index_dtype).tobytes()
     ptr = np.ascontiguousarray(m.ptr, index_dtype).tobytes()
     buf = val + idx + ptr
-    header = np.array([m.nnz, len(buf)], dtype=np.int64 if len(buf) > 2**31 - 1 else index_dtype).tobytes()
-    return header + buf
+    return wire_header(m.nnz, len(buf), index_dtype) + buf
 
 
 def tile_from_wire(data: bytes, nrows: int, ncols: int, index_dtype=np.int32, value_dtype=np.float64) -> Csr:
+    """Inverse of tile_to_wire (same index and value types on both sides)."""
     isz = np.dtype(index_dtype).itemsize
     vsz = np.dtype(value_dtype).itemsize
     nnz, buf_size = (int(x) for x in np.frombuffer(data[:2 * isz], dtype=index_dtype))
+    if len(data) < 2 * isz + buf_size:
+        raise ValueError("wire tile: message shorter than its header says")
+    if buf_size != nnz * (vsz + isz) + (nrows + 1) * isz:
+        raise ValueError("wire tile: buffer size does not match nnz and nrows")
     buf = data[2 * isz:2 * isz + buf_size]
     val = np.frombuffer(buf[:nnz * vsz], dtype=value_dtype).astype(np.float64)
     idx = np.frombuffer(buf[nnz * vsz:nnz * (vsz + isz)], dtype=index_dtype).astype(np.int32)
     ptr = np.frombuffer(buf[nnz * (vsz + isz):], dtype=index_dtype).astype(np.int32)
-    if ptr.shape[0] != nrows + 1:
-        raise ValueError("wire tile: pointer length does not match nrows")
     return Csr(nrows, ncols, ptr, idx, val)
