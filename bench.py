@@ -10,7 +10,14 @@ Default workload (one step = one plain C = A*A through tr_spgemm_panels):
 Prints ONE JSON line: GFLOP/s (2 * multiplications / step time), the
 dominant kernel's roofline (algorithmic HBM bytes / its HIP-event time vs
 8 TB/s), and a CPU baseline (the repository's FP64 oracle on a bounded row
-sample, which doubles as a bit-exactness check of those rows).
+sample, which doubles as a bit-exactness check of those rows, plus the
+Erdos-Renyi n=1024 product).  The same line carries the other forms, each
+plain vs batched/served with its dominant kernel's roofline and an oracle
+check on sampled rows:
+  batched   one SUMMA stage of R-MAT scale 22 cut 4x4 (stage 3), 16 tile
+            products added into partial-C tiles, one hipGraph launch;
+  galerkin  P^T A P on the 27-point Laplacian 128^3 with its SA prolongator;
+  served    2000 mixed requests at 1x and 4x the plain rate, p50/p99.
 
 Other workloads: --workload er1024 | rmat:<scale> (parity-sized cases).
 """
@@ -18,6 +25,7 @@ import argparse
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -111,36 +119,157 @@ def download_rows(ctx, panels, rb, hptrs, rows):
     return out
 
 
+def _cpu_topology():
+    """(physical cores, logical CPUs) among the CPUs this process may run on,
+    from the sysfs topology (unique package/core pairs)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("cpu", str(c)))
+    return max(1, len(cores)), len(cpus)
+
+
 def _cpu_model() -> str:
-    """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
+    """The host CPU's model name as lscpu reports it."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                return ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         names = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")]
-        return f"{names[0]} ({len(names)} logical CPUs visible)" if names else "unknown"
+        return names[0] if names else "unknown"
     except OSError:
         return "unknown"
 
 
-def cpu_baseline_and_check(A, f, ctx, panels, rb, hptrs, target_mults, seed=7):
-    """Time the oracle on a random row sample of ~target_mults multiplications
-    and compare those rows of C with the GPU result."""
+def cpu_env():
+    """Environment of the CPU-baseline child process: OpenMP threads = the
+    physical cores this process may use, bound close (BASELINE.md's
+    setting).  Binding is confined to the child, so it never pins this
+    process's own threads (the HIP runtime's, the server's workers)."""
+    phys, logical = _cpu_topology()
+    env = dict(os.environ, OMP_NUM_THREADS=str(phys), OMP_PROC_BIND="close")
+    return env, phys, logical
+
+
+def cpu_c1_baseline(oracle, repeats=10):
+    """The reference's CPU-runnable case: C = A*A, A = Erdos-Renyi n=1024 with
+    8 columns per row (seed 0), on the oracle: median wall time of `repeats`
+    runs after 2 warm-ups (the multiply only)."""
+    from trident_amd import workloads
+    A = workloads.product_operand("er1024")
+    mults = int(row_flops(A).sum())
+    ts = []
+    for r in range(repeats + 2):
+        t = time.perf_counter()
+        oracle.spgemm(A, A)
+        if r >= 2:
+            ts.append(time.perf_counter() - t)
+    t = float(np.median(ts))
+    return {"workload": "C = A*A, A = Erdos-Renyi n=1024, 8 columns/row (seed 0)", "mults": mults,
+            "ms": round(1000 * t, 4), "GFLOP/s": round(2 * mults / t / 1e9, 3)}
+
+
+def cpu_child(path):
+    """CPU-baseline child (bench.py --cpu-child PATH): reads the sampled rows
+    of A, B = A and the GPU's rows of C from PATH.npz, times the oracle on
+    them and on the Erdos-Renyi n=1024 product, checks the GPU rows, and
+    writes PATH.json."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from trident_amd import gen
+    z = np.load(path + ".npz")
+    S = gen.from_arrays(int(z["s_n"]), int(z["a_n"]), z["s_ptr"], z["s_idx"], z["s_val"])
+    A = gen.from_arrays(int(z["a_n"]), int(z["a_n"]), z["a_ptr"], z["a_idx"], z["a_val"])
+    t0 = time.perf_counter()
+    cp, ci, cv = oracle.spgemm(S, A)
+    t_cpu = time.perf_counter() - t0
+    gp = z["g_ptr"]
+    got = [(z["g_idx"][gp[i]:gp[i + 1]], z["g_val"][gp[i]:gp[i + 1]]) for i in range(len(gp) - 1)]
+    same, max_rel = compare_rows(got, cp, ci, cv)
+    out = {"t_cpu": t_cpu, "outputs": int(cp[-1]), "threads": oracle.num_threads(), "structure_equal": same,
+           "max_rel_err": max_rel, "c1": cpu_c1_baseline(oracle)}
+    with open(path + ".json", "w") as f:
+        json.dump(out, f)
+    return 0
+
+
+def cpu_baseline_and_check(A, f, ctx, panels, rb, hptrs, target_mults, seed=7):
+    """Time the oracle on a random row sample of ~target_mults multiplications
+    (in a child process with OpenMP bound to the physical cores) and compare
+    those rows of C with the GPU result."""
+    import tempfile
     rng = np.random.default_rng(seed)
     order = rng.permutation(A.nrows)
     cum = np.cumsum(f[order])
     k = int(np.searchsorted(cum, target_mults)) + 1
     rows = np.sort(order[:k])
-    sub_ptr = np.zeros(len(rows) + 1, np.int64)
-    lens = np.diff(A.ptr.astype(np.int64))[rows]
-    np.cumsum(lens, out=sub_ptr[1:])
-    sel = np.concatenate([np.arange(A.ptr[r], A.ptr[r + 1]) for r in rows]) if len(rows) else np.zeros(0, np.int64)
-    S = gen.from_arrays(len(rows), A.ncols, sub_ptr, A.idx[sel], A.val[sel])
+    S = take_rows(A, rows)
     mults = int(f[rows].sum())
-    t0 = time.perf_counter()
-    cp, ci, cv = oracle.spgemm(S, A)
-    t_cpu = time.perf_counter() - t0
     got = download_rows(ctx, panels, rb, hptrs, rows)
+    g_ptr = np.zeros(len(got) + 1, np.int64)
+    np.cumsum([g[0].shape[0] for g in got], out=g_ptr[1:])
+    env, phys, logical = cpu_env()
+    with tempfile.TemporaryDirectory() as td:
+        base = os.path.join(td, "cpu")
+        np.savez(base + ".npz", s_n=S.nrows, s_ptr=S.ptr, s_idx=S.idx, s_val=S.val, a_n=A.nrows, a_ptr=A.ptr,
+                 a_idx=A.idx, a_val=A.val, g_ptr=g_ptr,
+                 g_idx=np.concatenate([g[0] for g in got]) if got else np.zeros(0, np.int32),
+                 g_val=np.concatenate([g[1] for g in got]) if got else np.zeros(0))
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", base], env=env,
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"CPU baseline child failed: {r.stderr[-2000:]}")
+        with open(base + ".json") as fh:
+            res = json.load(fh)
+    t_cpu = res["t_cpu"]
+    return {
+        "value": round(2.0 * mults / t_cpu / 1e9, 3),
+        "unit": "GFLOP/s",
+        "cores": res["threads"],
+        "cpu": f"{_cpu_model()} ({phys} physical cores / {logical} logical CPUs available to the process; "
+               f"OMP_NUM_THREADS={env['OMP_NUM_THREADS']}, OMP_PROC_BIND={env['OMP_PROC_BIND']})",
+        "kind": "port",
+        "sample": (f"{len(rows)} random rows of A (seed {seed}) times A: {mults:.3e} multiplications, "
+                   f"{res['outputs']:.3e} outputs, {t_cpu:.2f} s; the repository's FP64 OpenMP Gustavson oracle, a "
+                   "substitute for the Kokkos OpenMP KokkosWrap path, which cannot be built here"),
+        "c1": res["c1"],
+    }, {"rows_checked": int(len(rows)), "structure_equal": res["structure_equal"], "max_rel_err": res["max_rel_err"]}
+
+
+def take_rows(A, rows):
+    """Host CSR of the given rows of A (all columns)."""
+    from trident_amd import gen
+    ptr = A.ptr.astype(np.int64)
+    lens = np.diff(ptr)[rows]
+    sub_ptr = np.zeros(len(rows) + 1, np.int64)
+    np.cumsum(lens, out=sub_ptr[1:])
+    sel = np.concatenate([np.arange(ptr[r], ptr[r + 1]) for r in rows]) if len(rows) else np.zeros(0, np.int64)
+    return gen.from_arrays(len(rows), A.ncols, sub_ptr, A.idx[sel], A.val[sel])
+
+
+def scatter_rows(n, ncols, rows, cp, ci, cv):
+    """Host CSR with n rows whose given rows hold (cp, ci, cv) and the rest
+    are empty."""
+    from trident_amd import gen
+    lens = np.zeros(n, np.int64)
+    lens[rows] = np.diff(cp)
+    ptr = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=ptr[1:])
+    return gen.from_arrays(n, ncols, ptr, ci, cv)
+
+
+def compare_rows(got, cp, ci, cv):
+    """got: list of (idx, val) per sampled row; (cp, ci, cv) the oracle's.
+    Returns (structure equal, max relative value error)."""
     same_struct = True
     max_rel = 0.0
     for t, (gi, gv) in enumerate(got):
@@ -152,39 +281,108 @@ def cpu_baseline_and_check(A, f, ctx, panels, rb, hptrs, target_mults, seed=7):
         if not np.array_equal(gv, ref):
             rel = np.abs(gv - ref) / np.maximum(np.abs(ref), np.finfo(np.float64).tiny)
             max_rel = max(max_rel, float(rel.max()))
-    return {
-        "value": round(2.0 * mults / t_cpu / 1e9, 3),
-        "unit": "GFLOP/s",
-        "cores": oracle.num_threads(),
-        "cpu": _cpu_model(),
-        "kind": "port",
-        "sample": (f"{len(rows)} random rows of A (seed {seed}) times A: {mults:.3e} multiplications, "
-                   f"{int(cp[-1]):.3e} outputs, {t_cpu:.2f} s; the repository's FP64 OpenMP Gustavson oracle "
-                   "(the Kokkos OpenMP KokkosWrap path cannot be built here)"),
-    }, {"rows_checked": int(len(rows)), "structure_equal": same_struct, "max_rel_err": max_rel}
+    return same_struct, max_rel
 
 
-def stage_bench(args):
-    """Batched stage (R-MAT scale S, 4x4 tiles, SUMMA stage k): the 16 tile
-    products C_ij <- C_ij + A_ik * A_kj as one graph launch vs 16 plain
-    sp_mma calls.  Partial-C tiles hold the same stage's products from a
-    previous pass, so the accumulate merges equal structures."""
-    from trident_amd import workloads
-    from trident_amd.device import Batch, Context, DeviceCsx, sp_mma, spgemm
-    _, scale, k = args.workload.split(":")
-    scale, k = int(scale), int(k)
-    g = 4
+def device_rows(ctx, M, rows):
+    """Rows of a device CSR tile (int32 ptr) as a list of (idx, val)."""
+    hp = ctx.to_host(M.d.ptr_vec, (M.d.nrows + 1,), np.int32).astype(np.int64)
+    out = []
+    for r in rows:
+        s, e = int(hp[r]), int(hp[r + 1])
+        out.append((ctx.to_host(M.d.idx_vec + 4 * s, (e - s,), np.int32),
+                    ctx.to_host(M.d.val + 8 * s, (e - s,), np.float64)))
+    return out
+
+
+def kernel_roofline(kt, steps, alg_bytes):
+    """Roofline entry of the dominant kernel in a per-kernel timing table
+    (name -> (launches, total ms)); alg_bytes(name) gives its algorithmic
+    bytes per launch and their definition, or None."""
+    if not kt:
+        return None
+    name, (launches, ms) = max(kt.items(), key=lambda kv: kv[1][1])
+    per_launch_s = ms / 1000.0 / max(launches, 1)
+    ab = alg_bytes(name, launches)
+    if ab is None:
+        return {"kernel": name, "kernel_ms_per_launch": round(1000 * per_launch_s, 3), "launches": launches}
+    alg, desc = ab
+    achieved = alg / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
+    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "alg_bytes_per_launch": int(alg), "alg_bytes_def": desc,
+            "kernel_ms_per_launch": round(1000 * per_launch_s, 3), "launches": launches}
+
+
+def use_tuning(workload, choice="auto"):
+    """Make the tuning of `workload` (the autotuner's JSON) the process default
+    for contexts created afterwards; returns (tuning, source)."""
+    from trident_amd import autotune
+    from trident_amd.device import load_tuning, set_default_tuning, tuning_default
+    set_default_tuning(None)
+    if choice == "none":
+        return tuning_default(), "built-in"
+    path = autotune.default_path(workload) if choice == "auto" else choice
+    if not os.path.exists(path):
+        return tuning_default(), "built-in"
+    t = dict(tuning_default(), **load_tuning(path))
+    set_default_tuning(t)
+    return t, os.path.relpath(path, ROOT)
+
+
+def class_bytes(alen, flops, out_nnz, b_bytes, cut):
+    """Algorithmic bytes of the light (flops <= cut) and heavy rows of one
+    product: 12 B per output entry written and per A entry read, plus B read
+    once by each class that has rows."""
+    light = (flops > 0) & (flops <= cut)
+    heavy = flops > cut
+    lb = 12 * int(out_nnz[light].sum()) + 12 * int(alen[light].sum()) + (b_bytes if light.any() else 0)
+    hb = 12 * int(out_nnz[heavy].sum()) + 12 * int(alen[heavy].sum()) + (b_bytes if heavy.any() else 0)
+    return lb, hb
+
+
+def stage_section(scale, k, steps, tuning="auto", with_compress=False, g=4):
+    """Batched stage.  R-MAT scale S (edge factor 16, seed 2) cut into a g x g
+    tile grid; SUMMA stage k updates every partial-C tile,
+    C_ij <- C_ij + A_ik * A_kj (16 uneven tile products).  The partial-C
+    tiles entering the stage are the products of stage k-1, A_i(k-1) A_(k-1)j:
+    the partial C of the ranks whose Cannon order reaches stage k right after
+    stage k-1; a tile whose sum would pass INT32_MAX entries (an int32 tile
+    cannot hold it) enters empty.  (The sum over all earlier stages does not
+    fit one GPU at scale 22: stage 0 alone makes about 5e10 entries.)  Plain: 16 sp_mma calls;
+    batched: one hipGraph launch.  Checked against the oracle on sampled rows
+    of three tiles."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from trident_amd import gen
+    from trident_amd.device import Batch, Context, DeviceCsx, sp_mma, spgemm, spgemm_flops, symbolic
+    tun, tsrc = use_tuning(f"stage:{scale}:{k}", tuning)
     t0 = time.perf_counter()
-    tiles = workloads.stage_tiles(scale, k, g)
-    log(f"[stage] tiles ready in {time.perf_counter() - t0:.1f}s")
+    A = gen.rmat(scale, 16, seed=2)
+    n = A.nrows // g
+    need = {(i, kk) for i in range(g) for kk in ({k, k - 1} if k > 0 else {k})}
+    need |= {(kk, j) for j in range(g) for kk in ({k, k - 1} if k > 0 else {k})}
+    T = {key: gen.submatrix(A, key[0] * n, (key[0] + 1) * n, key[1] * n, (key[1] + 1) * n) for key in need}
+    del A
+    log(f"[stage] R-MAT s{scale} tiles ready in {time.perf_counter() - t0:.1f}s")
     ctx = Context(0)
-    dT = {key: ctx.upload(m) for key, m in tiles.items()}
+    dT = {key: ctx.upload(m) for key, m in T.items()}
     keys = [(i, j) for i in range(g) for j in range(g)]
-    Cin = [spgemm(dT[(i, k)], dT[(k, j)]) for i, j in keys]
+    # a partial-C tile is an int32 tile (the reference's size_type): a tile
+    # whose previous-stage product plus this stage's would pass INT32_MAX
+    # entries enters the stage empty
+    Cin = []
+    for i, j in keys:
+        c = DeviceCsx(ctx)
+        if k > 0:
+            n_prev, _ = symbolic(dT[(i, k - 1)], dT[(k - 1, j)])
+            n_cur, _ = symbolic(dT[(i, k)], dT[(k, j)])
+            if n_prev + n_cur <= 2**31 - 1:
+                c = spgemm(dT[(i, k - 1)], dT[(k - 1, j)])
+        Cin.append(c)
     ctx.synchronize()
-    from trident_amd.device import spgemm_flops
     mults = sum(spgemm_flops(dT[(i, k)], dT[(k, j)]) for i, j in keys)
-    log(f"[stage] stage {k}: {mults:.4e} multiplications, partial C {sum(c.nnz for c in Cin):.4e} entries")
+    nnz_cin = sum(c.nnz for c in Cin)
+    log(f"[stage] stage {k}: {mults:.4e} multiplications, partial C in {nnz_cin:.4e} entries")
 
     def plain_step():
         outs = []
@@ -197,90 +395,119 @@ def stage_bench(args):
         ctx.synchronize()
         return outs
 
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+
     res = {}
-    for mode in ("plain", "batched", "batched+compress"):
-        if mode == "plain":
-            outs = plain_step()
-            for C in outs:
-                C.free()
-            ts = []
-            for _ in range(args.steps):
-                t = time.perf_counter()
-                outs = plain_step()
-                ts.append(time.perf_counter() - t)
-                if _ + 1 < args.steps:
-                    for C in outs:
-                        C.free()
-            plain_outs = outs
-        else:
-            b = Batch(ctx, [dT[(i, k)] for i, j in keys], [dT[(k, j)] for i, j in keys], Cin,
-                      compress=(mode == "batched+compress"))
+    for C in plain_step():  # warm-up
+        C.free()
+    holder = []
+
+    def plain_once():
+        while holder:
+            holder.pop().free()
+        holder.extend(plain_step())
+    t_plain = timed(plain_once, steps)
+    nnz_out = sum(c.nnz for c in holder)
+    while holder:
+        holder.pop().free()
+    res["plain"] = {"ms": round(1000 * t_plain, 2), "GFLOP/s": round(2 * mults / t_plain / 1e9, 2)}
+    log(f"[stage] plain: {res['plain']}")
+    modes = ["batched"] + (["batched+compress"] if with_compress else [])
+    rng = np.random.default_rng(11)
+    for mode in modes:
+        b = Batch(ctx, [dT[(i, k)] for i, j in keys], [dT[(k, j)] for i, j in keys], Cin,
+                  compress=(mode == "batched+compress"))
+
+        def run_b():
             b.run()
             ctx.synchronize()
-            ts = []
-            for _ in range(args.steps):
-                t = time.perf_counter()
-                b.run()
-                ctx.synchronize()
-                ts.append(time.perf_counter() - t)
-            # parity: batched tiles == plain tiles on sampled rows
-            rng = np.random.default_rng(1)
-            ok = True
-            for tt in range(len(keys)):
-                Hb, Hp = b.result(tt), plain_outs[tt]
-                pb = ctx.to_host(Hb.d.ptr_vec, (Hb.d.nrows + 1,), np.int32)
-                pp = ctx.to_host(Hp.d.ptr_vec, (Hp.d.nrows + 1,), np.int32)
-                ok &= bool(np.array_equal(pb, pp))
-                for r in rng.choice(Hb.d.nrows, 20, replace=False):
-                    s0, e0 = int(pb[r]), int(pb[r + 1])
-                    if e0 > s0:
-                        a1 = ctx.to_host(Hb.d.val + 8 * s0, (e0 - s0,), np.float64)
-                        a2 = ctx.to_host(Hp.d.val + 8 * s0, (e0 - s0,), np.float64)
-                        ok &= bool(np.array_equal(a1, a2))
-            res[mode + "_matches_plain"] = ok
-            b.close()
-        t_med = float(np.median(ts))
-        res[mode] = {"ms": round(1000 * t_med, 2), "GFLOP/s": round(2 * mults / t_med / 1e9, 2)}
+        run_b()
+        t_b = timed(run_b, steps)
+        b.check()
+        res[mode] = {"ms": round(1000 * t_b, 2), "GFLOP/s": round(2 * mults / t_b / 1e9, 2)}
         log(f"[stage] {mode}: {res[mode]}")
+        if mode == "batched":
+            # oracle on sampled rows of three tiles: the largest product, the
+            # smallest non-empty one and one more
+            fl = [spgemm_flops(dT[(i, k)], dT[(k, j)]) for i, j in keys]
+            order = [t for t in np.argsort(fl) if fl[t] > 0]
+            picks = sorted({int(order[-1]), int(order[0]), int(order[len(order) // 2])})
+            ok, max_rel, checked = True, 0.0, 0
+            for t in picks:
+                i, j = keys[t]
+                Ak, Bk = T[(i, k)], T[(k, j)]
+                rows = np.sort(rng.choice(Ak.nrows, 48, replace=False))
+                P = oracle.spgemm(take_rows(Ak, rows), Bk)
+                if Cin[t].initialized:
+                    Q = oracle.spgemm(take_rows(T[(i, k - 1)], rows), T[(k - 1, j)])
+                    Pm = gen.from_arrays(len(rows), Bk.ncols, *P)
+                    Qm = gen.from_arrays(len(rows), Bk.ncols, *Q)
+                    P = oracle.spadd(Pm, Qm)
+                same, rel = compare_rows(device_rows(ctx, b.result(t), rows), *P)
+                ok &= same
+                max_rel = max(max_rel, rel)
+                checked += len(rows)
+            res["oracle_sample"] = {"tiles": [list(keys[t]) for t in picks], "rows_checked": checked,
+                                    "structure_equal": bool(ok), "max_rel_err": max_rel}
+            log(f"[stage] oracle sample: {res['oracle_sample']}")
+            _, nnz_p, _ = b.stats()
+        b.close()
     res["speedup_batched_vs_plain"] = round(res["plain"]["ms"] / res["batched"]["ms"], 3)
-    # per-kernel HIP-event times: one plain pass and one eager (uncaptured) batch pass
-    for mode in ("plain", "batched_eager"):
-        ctx.reset_timing()
-        ctx.set_timing(True)
-        if mode == "plain":
-            for C in plain_step():
-                C.free()
-        else:
-            b = Batch(ctx, [dT[(i, k)] for i, j in keys], [dT[(k, j)] for i, j in keys], Cin, compress=False,
-                      graph=False)
-            ctx.synchronize()
-            ctx.reset_timing()
-            b.run()
-            ctx.synchronize()
-            b.close()
-        ctx.set_timing(False)
-        res[mode + "_kernels_ms"] = {kk: round(v[1], 3) for kk, v in sorted(ctx.timings().items())}
-        log(f"[stage] {mode} kernels: {res[mode + '_kernels_ms']}")
-    out = {
-        "metric": METRIC, "value": res["batched"]["GFLOP/s"], "unit": "GFLOP/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": 1, "ms_per_step": res["batched"]["ms"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"R-MAT scale {scale} (ef 16, seed 2) cut 4x4; SUMMA stage k={k}: 16 tile products "
-                               "accumulated into partial-C tiles", "mults": mults, "forms": res,
-                   "tuning": args.tuning_used, "tuning_source": args.tuning_source},
-    }
-    print(json.dumps(out), flush=True)
-    return 0
+    # per-kernel HIP-event times of one eager pass of the batch
+    b = Batch(ctx, [dT[(i, k)] for i, j in keys], [dT[(k, j)] for i, j in keys], Cin, compress=False, graph=False)
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    b.run()
+    ctx.synchronize()
+    ctx.set_timing(False)
+    kt = ctx.timings()
+    b.close()
+    rows_total = g * n
+    ptr_b = 4 * 3 * (rows_total + g * g)
+
+    def alg(name, launches):
+        if name == "add_write":
+            return (12 * (nnz_p + nnz_cin + nnz_out) + ptr_b,
+                    "12 B per product entry and per partial-C entry read, per output entry written, + row pointers")
+        if name == "add_count":
+            return (4 * (nnz_p + nnz_cin) + ptr_b, "4 B per product and partial-C column index read + row pointers")
+        return None
+    res["kernels_ms"] = {kk: round(v[1], 3) for kk, v in sorted(kt.items())}
+    res["roofline"] = kernel_roofline(kt, 1, alg)
+    tile_bytes = sum(12 * m.nnz + 4 * (m.nrows + 1) for key, m in T.items() if k in key)
+    step_bytes = tile_bytes + 12 * (nnz_cin + nnz_out) + 8 * (rows_total + g * g)
+    res["step_alg_GBs"] = round(step_bytes / (res["batched"]["ms"] / 1000) / 1e9, 2)
+    res["step_frac"] = round(res["step_alg_GBs"] / HBM_PEAK_GBS, 5)
+    res.update({"workload": f"R-MAT scale {scale} (ef 16, seed 2) cut {g}x{g}; SUMMA stage k={k}: {g * g} tile "
+                            f"products added into partial-C tiles (the stage {k - 1} products)",
+                "mults": mults, "nnz_partial_C_in": nnz_cin, "tiles_with_partial_C": sum(c.initialized for c in Cin),
+                "nnz_products": nnz_p, "nnz_out": nnz_out,
+                "tuning": tun, "tuning_source": tsrc})
+    for c in Cin:
+        if c.initialized:
+            c.free()
+    ctx.close()
+    return res
 
 
-def galerkin_bench(args):
+def galerkin_section(n, steps, tuning="auto"):
     """AMG Galerkin product C = P^T (A P): 27-point Laplacian on n^3 (26/-1),
     smoothed-aggregation P (3^3 aggregates, damped Jacobi w = 2/3).  Plain:
     two tr_spgemm calls; batched: the two products as captured graphs (P^T
-    from the on-device transpose, resident)."""
+    from the on-device transpose, resident).  Checked against the oracle on
+    sampled rows of C."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
     from trident_amd import workloads
     from trident_amd.device import Batch, Context, spgemm, spgemm_flops
-    n = int(args.workload.split(":")[1])
+    tun, tsrc = use_tuning(f"galerkin:{n}", tuning)
     t0 = time.perf_counter()
     A, P = workloads.galerkin_operators(n)
     log(f"[galerkin] operators ready in {time.perf_counter() - t0:.1f}s: A {A.nrows} x {A.ncols} nnz {A.nnz}, "
@@ -300,6 +527,9 @@ def galerkin_bench(args):
     C = spgemm(dPt, AP)
     ctx.synchronize()
     ref = C.download()
+    APh = AP.download()
+    AP.free()
+    C.free()
 
     def plain():
         ap = spgemm(dA, dP)
@@ -308,69 +538,128 @@ def galerkin_bench(args):
         return ap, c
     ap, c = plain()
     ts = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         ap.free(), c.free()
         t = time.perf_counter()
         ap, c = plain()
         ts.append(time.perf_counter() - t)
+    ap.free(), c.free()
     t_plain = float(np.median(ts))
     b1 = Batch(ctx, [dA], [dP])
     ap_b = b1.result(0)
     b2 = Batch(ctx, [dPt], [ap_b])
     ts = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         t = time.perf_counter()
         b1.run()
         b2.run()
         ctx.synchronize()
         ts.append(time.perf_counter() - t)
+    b1.check()
+    b2.check()
     t_b = float(np.median(ts))
     got = b2.result(0).download()
     ok = bool(np.array_equal(got.ptr, ref.ptr) and np.array_equal(got.idx, ref.idx) and np.array_equal(got.val, ref.val))
-    res = {"transpose_ms": round(1000 * t_tr, 3), "transpose_kernels_ms": tr_kernels,
+    # oracle on sampled rows of C: the AP rows they need, then those rows of P^T (AP)
+    rng = np.random.default_rng(5)
+    Pt = oracle.transpose(P)
+    from trident_amd import gen
+    Pt = gen.from_arrays(P.ncols, P.nrows, *Pt)
+    rows = np.sort(rng.choice(Pt.nrows, 64, replace=False))
+    needed = np.unique(np.concatenate([Pt.idx[Pt.ptr[r]:Pt.ptr[r + 1]] for r in rows]))
+    APn = oracle.spgemm(take_rows(A, needed), P)
+    APs = scatter_rows(A.nrows, P.ncols, needed, *APn)
+    Cr = oracle.spgemm(take_rows(Pt, rows), APs)
+    gi = [(got.idx[got.ptr[r]:got.ptr[r + 1]], got.val[got.ptr[r]:got.ptr[r + 1]]) for r in rows]
+    same, max_rel = compare_rows(gi, *Cr)
+    # per-kernel HIP-event times of one eager pass of both products
+    e1 = Batch(ctx, [dA], [dP], graph=False)
+    e2 = Batch(ctx, [dPt], [e1.result(0)], graph=False)
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    e1.run()
+    e2.run()
+    ctx.synchronize()
+    ctx.set_timing(False)
+    kt = ctx.timings()
+    e2.close()
+    e1.close()
+    cut = tun["light_max_flops"]
+    # per-row classes of both products
+    alen_a, alen_pt = np.diff(A.ptr.astype(np.int64)), np.diff(Pt.ptr.astype(np.int64))
+    plen, aplen = np.diff(P.ptr.astype(np.int64)), np.diff(APh.ptr.astype(np.int64))
+    f1 = np.zeros(A.nrows, np.int64)
+    np.add.at(f1, A.row_ids(), plen[A.idx])
+    f2 = np.zeros(Pt.nrows, np.int64)
+    np.add.at(f2, Pt.row_ids(), aplen[Pt.idx])
+    l1, h1 = class_bytes(alen_a, f1, aplen, 12 * P.nnz + 4 * (P.nrows + 1), cut)
+    l2, h2 = class_bytes(alen_pt, f2, np.diff(got.ptr.astype(np.int64)), 12 * APh.nnz + 4 * (APh.nrows + 1), cut)
+
+    def alg(name, launches):
+        if name == "num_light":
+            return ((l1 + l2) / max(launches, 1), "light rows of both products: 12 B per C entry written and per "
+                                                  "A entry read + B read once per product")
+        if name == "num_heavy":
+            return ((h1 + h2) / max(launches, 1), "heavy rows of both products: 12 B per C entry written and per "
+                                                  "A entry read + B read once per product")
+        return None
+    step_bytes = (12 * (A.nnz + P.nnz) + 4 * (A.nrows + P.nrows + 2) + 12 * (Pt.nnz + APh.nnz) +
+                  4 * (Pt.nrows + APh.nrows + 2) + 12 * (APh.nnz + got.nnz) + 4 * (APh.nrows + got.nrows + 2))
+    res = {"workload": f"C = P^T A P, A = 27-point Laplacian {n}^3 (26/-1), P = SA prolongator (3^3 aggregates, "
+                       "unnormalised tentative P, w = 2/3)", "mults": mults,
+           "transpose_ms": round(1000 * t_tr, 3), "transpose_kernels_ms": tr_kernels,
            "plain": {"ms": round(1000 * t_plain, 3), "GFLOP/s": round(2 * mults / t_plain / 1e9, 2)},
            "batched": {"ms": round(1000 * t_b, 3), "GFLOP/s": round(2 * mults / t_b / 1e9, 2)},
-           "batched_matches_plain": ok, "nnz_AP": ap_b.nnz, "nnz_C": got.nnz}
-    log(f"[galerkin] {res}")
-    out = {"metric": METRIC, "value": res["batched"]["GFLOP/s"], "unit": "GFLOP/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": 1, "ms_per_step": res["batched"]["ms"], "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"C = P^T A P, A = 27-point Laplacian {n}^3 (26/-1), P = SA prolongator "
-                                  "(3^3 aggregates, unnormalised tentative P, w = 2/3)", "mults": mults, "forms": res,
-                      "tuning": args.tuning_used, "tuning_source": args.tuning_source}}
-    print(json.dumps(out), flush=True)
-    return 0
+           "speedup_batched_vs_plain": round(t_plain / t_b, 3),
+           "batched_matches_plain": ok, "nnz_AP": APh.nnz, "nnz_C": got.nnz,
+           "oracle_sample": {"rows_checked": int(len(rows)), "structure_equal": bool(same), "max_rel_err": max_rel},
+           "kernels_ms": {kk: round(v[1], 3) for kk, v in sorted(kt.items())},
+           "roofline": kernel_roofline(kt, 1, alg),
+           "step_alg_GBs": round(step_bytes / t_b / 1e9, 2), "step_frac": round(step_bytes / t_b / 1e9 / HBM_PEAK_GBS, 5),
+           "tuning": tun, "tuning_source": tsrc}
+    log(f"[galerkin] plain {res['plain']} batched {res['batched']} oracle {res['oracle_sample']}")
+    b2.close()
+    b1.close()
+    ctx.close()
+    return res
 
 
-def serve_bench(args):
-    """Served stream of mixed requests with Poisson arrivals: 60% A*A on R-MAT
-    scales 12-18 (edge factor 8 or 16), 40% P^T A P on 27-point Laplacians
-    16^3-96^3.  Reports p50/p99 latency and throughput per arrival rate, and
-    the plain aggregate (each request alone, one after another)."""
-    import threading
+def serve_section(nreq, rates, budget_gb, workers, tuning="auto"):
+    """Served stream of `nreq` mixed requests with Poisson arrivals: 60% A*A on
+    R-MAT scales 12-18 (edge factor 8 or 16), 40% P^T A P on 27-point
+    Laplacians 16^3-96^3 (seed 3).  For each arrival rate (a multiple of what
+    the plain path sustains: every request alone, one after another) it
+    reports p50/p99 latency, throughput and the ratio to plain; probe
+    requests are checked against the oracle."""
+    import threading  # noqa: F401  (the server's workers are native threads)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
     from trident_amd import gen
     from trident_amd._lib import TridentError
-    from trident_amd.device import (Context, Server, TR_REQ_GALERKIN, TR_REQ_SPGEMM, spgemm, spgemm_flops)
-    parts = args.workload.split(":")
-    nreq = int(parts[1]) if len(parts) > 1 else 2000
+    from trident_amd.device import (Context, DeviceCsx, Server, TR_REQ_GALERKIN, TR_REQ_SPGEMM, spgemm,
+                                    spgemm_flops)
+    tun, tsrc = use_tuning("serve", tuning)
     rng = np.random.default_rng(3)
     t0 = time.perf_counter()
     ctx = Context(0)
-    rm = {}
+    host_rm, rm = {}, {}
     for s_ in range(12, 19):
         for ef in (8, 16):
-            rm[(s_, ef)] = ctx.upload(gen.rmat(s_, ef, seed=100 * s_ + ef))
-    lp = {}
+            host_rm[(s_, ef)] = gen.rmat(s_, ef, seed=100 * s_ + ef)
+            rm[(s_, ef)] = ctx.upload(host_rm[(s_, ef)])
+    host_lp, lp = {}, {}
     for n in (16, 32, 48, 64, 80, 96):
-        lp[n] = (ctx.upload(gen.laplace27(n)), ctx.upload(gen.sa_prolongator(n)))
+        host_lp[n] = (gen.laplace27(n), gen.sa_prolongator(n))
+        lp[n] = (ctx.upload(host_lp[n][0]), ctx.upload(host_lp[n][1]))
     log(f"[serve] operands ready in {time.perf_counter() - t0:.1f}s")
     # plain reference: every request type alone, one after another
     kinds = [("spgemm", k) for k in rm] + [("galerkin", n) for n in lp]
     plain_t, flops = {}, {}
     for kind, key in kinds:
         for rep in range(2):
-            t = time.perf_counter()
             if kind == "spgemm":
+                t = time.perf_counter()
                 c = spgemm(rm[key], rm[key])
                 ctx.synchronize()
                 dt = time.perf_counter() - t
@@ -401,12 +690,12 @@ def serve_bench(args):
     tot_plain = sum(plain_t[r] for r in reqs)
     plain_gflops = 2 * tot_flops / tot_plain / 1e9
     log(f"[serve] plain: {plain_gflops:.2f} GFLOP/s over the mix ({tot_plain:.1f} s sequential)")
-    srv = Server(mem_budget=int(args.serve_budget_gb * (1 << 30)), workers=args.serve_workers, max_batch=32)
+    srv = Server(mem_budget=int(budget_gb * (1 << 30)), workers=workers, max_batch=32)
     ops = {k: srv.register(v) for k, v in rm.items()}
     lops = {n: (srv.register(A), srv.register(P)) for n, (A, P) in lp.items()}
     capacity = nreq / tot_plain  # requests/s the plain path sustains
     results = []
-    for factor in args.serve_rates:
+    for factor in rates:
         rate = factor * capacity
         gaps = rng.exponential(1.0 / rate, size=nreq)
         srv.reset_stats()
@@ -440,15 +729,65 @@ def serve_bench(args):
         st = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
         log(f"[serve] {st}")
         results.append(st)
+    # probes: served results of small requests against the oracle
+    probes = [("spgemm", (12, 8)), ("spgemm", (13, 16)), ("galerkin", 16)]
+    pid = []
+    for kind, key in probes:
+        if kind == "spgemm":
+            pid.append(srv.submit(TR_REQ_SPGEMM, ops[key], ops[key]))
+        else:
+            pid.append(srv.submit(TR_REQ_GALERKIN, lops[key][0], lops[key][1]))
+    probe_ok = True
+    for (kind, key), rid in zip(probes, pid):
+        d, _ = srv.wait(rid)
+        H = DeviceCsx(ctx, d).download()
+        if kind == "spgemm":
+            ep, ei, ev = oracle.spgemm(host_rm[key], host_rm[key])
+        else:
+            Ah, Ph = host_lp[key]
+            APo = gen.from_arrays(Ah.nrows, Ph.ncols, *oracle.spgemm(Ah, Ph))
+            Pto = gen.from_arrays(Ph.ncols, Ph.nrows, *oracle.transpose(Ph))
+            ep, ei, ev = oracle.spgemm(Pto, APo)
+        probe_ok &= bool(np.array_equal(H.ptr.astype(np.int64), ep) and np.array_equal(H.idx, ei)
+                         and np.array_equal(H.val.view(np.uint64), ev.view(np.uint64)))
+        srv.release(rid)
     srv.close()
+    ctx.close()
     best = max(results, key=lambda r: r["served_GFLOP/s"])
-    out = {"metric": METRIC, "value": best["served_GFLOP/s"], "unit": "GFLOP/s", "n_gpus": 1, "steps": nreq,
-           "warmup": 0, "ms_per_step": round(1000 * best["wall_s"] / nreq, 3), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"{nreq} requests, Poisson arrivals: 60% A*A on R-MAT s12-18 (ef 8/16), 40% "
-                                  "P^T A P on 27-pt Laplacians 16^3-96^3", "plain_GFLOP/s": round(plain_gflops, 2),
-                      "rates": results, "tuning": args.tuning_used, "tuning_source": args.tuning_source}}
-    print(json.dumps(out), flush=True)
+    return {"workload": f"{nreq} requests, Poisson arrivals: 60% A*A on R-MAT s12-18 (ef 8/16), 40% P^T A P on "
+                        "27-pt Laplacians 16^3-96^3", "requests": nreq, "plain_GFLOP/s": round(plain_gflops, 2),
+            "best_served_GFLOP/s": best["served_GFLOP/s"], "best_vs_plain": best["vs_plain"], "rates": results,
+            "oracle_probes": {"requests": [f"{k}:{v}" for k, v in probes], "bit_exact": probe_ok},
+            "workers": workers, "tuning": tun, "tuning_source": tsrc}
+
+
+def section_line(args, res, value_key, ms_key, steps):
+    """The one-line JSON of a form run on its own (--workload stage:/galerkin:/serve)."""
+    return {"metric": METRIC, "value": value_key, "unit": "GFLOP/s", "n_gpus": 1, "steps": steps,
+            "warmup": 1, "ms_per_step": ms_key, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic", "config": {"workload": res["workload"], "forms": res}}
+
+
+def stage_bench(args):
+    _, scale, k = args.workload.split(":")
+    res = stage_section(int(scale), int(k), args.steps, args.tuning, with_compress=True)
+    print(json.dumps(section_line(args, res, res["batched"]["GFLOP/s"], res["batched"]["ms"], args.steps)), flush=True)
+    return 0
+
+
+def galerkin_bench(args):
+    res = galerkin_section(int(args.workload.split(":")[1]), args.steps, args.tuning)
+    print(json.dumps(section_line(args, res, res["batched"]["GFLOP/s"], res["batched"]["ms"], args.steps)), flush=True)
+    return 0
+
+
+def serve_bench(args):
+    parts = args.workload.split(":")
+    nreq = int(parts[1]) if len(parts) > 1 else 2000
+    res = serve_section(nreq, args.serve_rates, args.serve_budget_gb, args.serve_workers, args.tuning)
+    best = max(res["rates"], key=lambda r: r["served_GFLOP/s"])
+    print(json.dumps(section_line(args, res, best["served_GFLOP/s"], round(1000 * best["wall_s"] / nreq, 3), nreq)),
+          flush=True)
     return 0
 
 
@@ -488,10 +827,17 @@ def main():
     ap.add_argument("--cpu-mults", type=float, default=1.5e9,
                     help="multiplications in the CPU-baseline row sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-child", default="", help=argparse.SUPPRESS)
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
     ap.add_argument("--serve-workers", type=int, default=1,
                     help="server worker threads (one batch at a time keeps the GPU's caches for one product)")
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.5, 1.0, 4.0])
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[1.0, 4.0])
+    ap.add_argument("--sections", default="all",
+                    help="forms measured after the plain product on its default workload: all, or none, or a "
+                         "comma list of batched,galerkin,served")
+    ap.add_argument("--stage", default="22:3", help="batched form: R-MAT scale and SUMMA stage, S:k")
+    ap.add_argument("--galerkin", type=int, default=128, help="Galerkin form: Laplacian edge n (n^3 grid)")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="served form: requests in the stream")
     ap.add_argument("--tuning", default="auto",
                     help="auto (trident_amd/tuning/<workload>.json if present), none, or a JSON path")
     args = ap.parse_args()
@@ -499,6 +845,8 @@ def main():
         print(f"bench.py: this build measures one GPU's share of the path; --gpus {args.gpus} is not supported "
               "(run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.cpu_child:
+        return cpu_child(args.cpu_child)
     if args.workload.startswith("copy:"):
         return copy_bench(args)
     args.tuning_used, args.tuning_source = apply_tuning(args)
@@ -645,6 +993,21 @@ def main():
     }
     for P in panels:
         P.free()
+    ctx.close()
+    # the batched, Galerkin and served forms, each with its dominant kernel's
+    # roofline and an oracle check on sampled rows
+    want = set() if args.sections == "none" else (
+        {"batched", "galerkin", "served"} if args.sections == "all" else set(args.sections.split(",")))
+    if args.workload != "rmat:20":
+        want = set() if args.sections == "all" else want
+    if "batched" in want:
+        sc, kk = (int(x) for x in args.stage.split(":"))
+        out["batched"] = stage_section(sc, kk, args.steps, args.tuning)
+    if "galerkin" in want:
+        out["galerkin"] = galerkin_section(args.galerkin, args.steps, args.tuning)
+    if "served" in want:
+        out["served"] = serve_section(args.serve_requests, args.serve_rates, args.serve_budget_gb,
+                                      args.serve_workers, args.tuning)
     print(json.dumps(out), flush=True)
     return 0
 
