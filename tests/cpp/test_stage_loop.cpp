@@ -18,6 +18,7 @@
 // the oracle's stage products summed in the same order.
 //
 // usage: test_stage_loop <scale> <edge_factor> <seed> <P> <mode> <out_file>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -123,6 +124,7 @@ int main(int argc, char **argv) {
     FILE *out = std::fopen(argv[6], "wb");
     if (!out) return 2;
     KokkosWrap::Handle handle;
+    double loop_ms = 0.0;  // the ranks' stage loops (multiplies, adds, tile movement), summed
     for (int r = 0; r < P; ++r)
         for (int c = 0; c < P; ++c) {
             // buffers of the rank (hns_spgemm.cu:640-650)
@@ -144,6 +146,8 @@ int main(int argc, char **argv) {
             if (mode == 2) accum_handle = new AccumThreadHandle<IT, VT>(&C_lm, P, 0);
             if (accum_handle) accum_thread = std::thread([&] { accum_thread_loop(*accum_handle); });
 
+            check(tr_ctx_synchronize(ctx), "synchronize");
+            const auto t_loop = std::chrono::steady_clock::now();
             bool did_one_spgemm = false;
             int colAtoGet = (r + c) % P;  // stagger left (hns_spgemm.cu:587)
             int rowBtoGet = (c + r) % P;  // stagger down (:588)
@@ -191,6 +195,8 @@ int main(int argc, char **argv) {
                 rowBtoGet = (rowBtoGet + 1) % P;  // shift down (:1001)
             }
             if (accum_handle) accum_thread.join();
+            check(tr_ctx_synchronize(ctx), "synchronize");
+            loop_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_loop).count();
 
             // the rank's partial C
             tr_csx_t C{};
@@ -223,6 +229,7 @@ int main(int argc, char **argv) {
     for (auto *s : spacomm) delete s;
     for (auto &d : dA) tr_csx_free(ctx, &d);
     for (auto &d : dB) tr_csx_free(ctx, &d);
+    std::printf("stage loops: %.3f ms (mode %d)\n", loop_ms, mode);
     std::printf("ALL OK\n");
     return 0;
 }
