@@ -217,6 +217,18 @@ int tr_batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_cs
                     int32_t flags, tr_batch_t *batch);
 int tr_batch_run(tr_batch_t batch);
 int tr_batch_result(tr_batch_t batch, int32_t task, tr_csx_t *C_out);
+/* Task t's result as consecutive row panels of at most max_panel_nnz
+ * (<= INT32_MAX) entries each: the way to read a result that is larger than
+ * an int32 tile (possible for a batch without accumulators, whose results
+ * are its products; tr_batch_result returns TR_ERR_OVERFLOW for such a
+ * task).  Panel p holds rows [row_begin[p], row_begin[p+1]) of the result;
+ * the panels are views into the batch's storage with int32 row pointers of
+ * their own, valid until tr_batch_destroy.  Uses the batch's context: call
+ * it from the thread that owns the context.  No counterpart in the
+ * reference, whose int32 size_type cannot hold such a tile
+ * (include/KokkosWrap.hpp:46); it is tr_spgemm_panels for a batch. */
+int tr_batch_result_panels(tr_batch_t batch, int32_t task, int64_t max_panel_nnz, tr_csx_t *panels,
+                           int32_t *row_begin, int32_t max_panels, int32_t *npanels);
 int tr_batch_check(tr_batch_t batch);
 int tr_batch_stats(tr_batch_t batch, int64_t *flops, int64_t *nnz_products, int64_t *nnz_out);
 int tr_batch_destroy(tr_batch_t batch);

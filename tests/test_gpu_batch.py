@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 from trident_amd import gen
-from trident_amd._lib import TR_ERR_INVALID, TridentError
+from trident_amd._lib import TR_ERR_INVALID, TR_ERR_OVERFLOW, TridentError
 from trident_amd.device import Batch, sp_mma, DeviceCsx
 
 pytestmark = pytest.mark.gpu
@@ -163,4 +163,40 @@ def test_replay_guard_rejects_changed_pattern(ctx, graph, accum):
     b.run()
     b.check()
     assert same(b.result(0).download(), ref_add(prod(A, Bv), C0))
+    b.close()
+
+
+@pytest.mark.parametrize("accum", [False, True])
+def test_result_panels_match_oracle(ctx, accum):
+    """A batch result read as int32 row panels (the way a result larger than
+    an int32 tile is read) under a forced small panel cap: the panels
+    concatenate to the oracle's result and each holds at most the cap."""
+    A = gen.rmat(11, 16, seed=4)
+    T = tiles_of(A, 2)
+    keys = [(0, 0), (1, 1), (0, 1)]
+    dT = {key: ctx.upload(m) for key, m in T.items()}
+    Cin_h = [prod(T[(i, 1)], T[(1, j)]) if accum else None for i, j in keys]
+    Cin_d = [ctx.upload(c) for c in Cin_h] if accum else None
+    print(f"result panels: accum={accum}", flush=True)
+    b = Batch(ctx, [dT[(i, 0)] for i, j in keys], [dT[(0, j)] for i, j in keys], Cin_d)
+    for t, (i, j) in enumerate(keys):
+        R = ref_add(prod(T[(i, 0)], T[(0, j)]), Cin_h[t])
+        lens = np.diff(R.ptr.astype(np.int64))
+        cap = max(int(lens.max()), R.nnz // 5)
+        panels, rb = b.result_panels(t, max_panel_nnz=cap)
+        assert rb[0] == 0 and rb[-1] == R.nrows and len(panels) >= 2
+        ptr, idx, val = [np.zeros(1, np.int64)], [], []
+        for P in panels:
+            H = P.download()
+            assert H.nnz <= cap
+            ptr.append(ptr[-1][-1] + H.ptr[1:].astype(np.int64))
+            idx.append(H.idx)
+            val.append(H.val)
+        assert np.array_equal(np.concatenate(ptr), R.ptr.astype(np.int64))
+        assert np.array_equal(np.concatenate(idx), R.idx)
+        assert np.array_equal(np.concatenate(val).view(np.uint64), R.val.view(np.uint64))
+        if lens.max() > 1:
+            with pytest.raises(TridentError) as e:
+                b.result_panels(t, max_panel_nnz=int(lens.max()) - 1)
+            assert e.value.code == TR_ERR_OVERFLOW
     b.close()

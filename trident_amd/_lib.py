@@ -85,6 +85,7 @@ SIGNATURES = [
     ("tr_batch_create", ctypes.c_int, [_vp, _i32, _csx, _csx, _csx, _i32, _P(_vp)]),
     ("tr_batch_run", ctypes.c_int, [_vp]),
     ("tr_batch_result", ctypes.c_int, [_vp, _i32, _csx]),
+    ("tr_batch_result_panels", ctypes.c_int, [_vp, _i32, _i64, _csx, _P(_i32), _i32, _P(_i32)]),
     ("tr_batch_check", ctypes.c_int, [_vp]),
     ("tr_batch_stats", ctypes.c_int, [_vp, _P(_i64), _P(_i64), _P(_i64)]),
     ("tr_batch_destroy", ctypes.c_int, [_vp]),

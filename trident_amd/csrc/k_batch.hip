@@ -322,14 +322,17 @@ void batch_body(tr_batch *b, bool capture) {
         hp.val = b->p_val;
         b->d_panel = (Panel *)b->own(sizeof(Panel));
         TR_HIP(hipMemcpyAsync(b->d_panel, &hp, sizeof(Panel), hipMemcpyHostToDevice, ctx->stream));
-        // per-task product sizes must fit int32 tiles
+        // a product added into a partial-C tile must fit an int32 tile; a
+        // product that is itself the result may be larger (it is then read
+        // as row panels, tr_batch_result_panels)
         std::vector<int64_t> rp(b->nt + 1);
         for (int t = 0; t <= b->nt; ++t) {
             const int32_t g = t < b->nt ? b->mp.h_tasks[t].row_base : b->mp.rows_total;
             rp[t] = read_scalar<int64_t>(ctx, b->mp.rowptr + g);
         }
         for (int t = 0; t < b->nt; ++t)
-            if (rp[t + 1] - rp[t] > INT32_MAX) fail(TR_ERR_OVERFLOW, "batch: a tile product exceeds INT32_MAX entries");
+            if ((!b->no_accum || b->single_run) && rp[t + 1] - rp[t] > INT32_MAX)
+                fail(TR_ERR_OVERFLOW, "batch: a tile product exceeds INT32_MAX entries");
     }
     mul_numeric(ctx, b->mp, b->d_panel, 1);
     if (!capture) {
@@ -534,7 +537,8 @@ int tr::batch_create(tr_ctx_t ctx, int32_t ntasks, const tr_csx_t *A, const tr_c
         for (int t = 0; t < ntasks; ++t)
             b->o_nnz[t] = (t + 1 < ntasks ? b->o_base[t + 1] : b->o_total) - b->o_base[t];
         for (int t = 0; t < ntasks; ++t)
-            if (b->o_nnz[t] > INT32_MAX) fail(TR_ERR_OVERFLOW, "batch: an accumulated tile exceeds INT32_MAX entries");
+            if (!b->no_accum && b->o_nnz[t] > INT32_MAX)
+                fail(TR_ERR_OVERFLOW, "batch: an accumulated tile exceeds INT32_MAX entries");
         b->ran = true;  // the eager pass already produced the results
         if (!b->single_run) {
             // what a replay must reproduce: operand ends and task offsets
@@ -620,6 +624,7 @@ int tr_batch_run(tr_batch_t b) {
 // owner keeps using the context.
 int tr_batch_result(tr_batch_t b, int32_t t, tr_csx_t *C_out) {
     if (!b || !C_out || t < 0 || t >= b->nt) return TR_ERR_INVALID;
+    if (b->o_nnz[t] > INT32_MAX) return TR_ERR_OVERFLOW;  // read it with tr_batch_result_panels
     tr_csx_t c{};
     c.majordim = TR_ROWS;
     c.nrows = b->A[t].nrows;
@@ -649,6 +654,69 @@ size_t tr::batch_retained_bytes(tr_batch_t b) {
 }
 
 extern "C" {
+
+int tr_batch_result_panels(tr_batch_t b, int32_t t, int64_t max_panel_nnz, tr_csx_t *panels, int32_t *row_begin,
+                           int32_t max_panels, int32_t *npanels) {
+    if (!b) return TR_ERR_INVALID;
+    tr_ctx *ctx = b->ctx;
+    return bguard(ctx, [&] {
+        TR_REQUIRE(t >= 0 && t < b->nt && panels && row_begin && npanels && max_panels >= 1,
+                   "tr_batch_result_panels: bad arguments");
+        TR_REQUIRE(max_panel_nnz >= 1 && max_panel_nnz <= INT32_MAX,
+                   "tr_batch_result_panels: max_panel_nnz must be in [1, INT32_MAX]");
+        const int32_t m = b->A[t].nrows;
+        // 64-bit row pointers of the task's result: the product's (no
+        // accumulators) or, for an accumulated tile (always an int32 tile),
+        // its own row pointers widened
+        std::vector<int64_t> rp((size_t)m + 1);
+        if (b->no_accum) {
+            read_array(ctx, rp.data(), b->mp.rowptr + b->mp.h_tasks[t].row_base, sizeof(int64_t) * ((size_t)m + 1));
+        } else {
+            std::vector<int32_t> r32((size_t)m + 1);
+            read_array(ctx, r32.data(), b->o_ptr + b->optr_off[t], sizeof(int32_t) * ((size_t)m + 1));
+            for (size_t i = 0; i <= (size_t)m; ++i) rp[i] = b->o_base[t] + r32[i];
+        }
+        std::vector<int32_t> rb{0};
+        int64_t start = rp[0];
+        for (int32_t r = 0; r < m; ++r) {
+            if (rp[r + 1] - rp[r] > max_panel_nnz) fail(TR_ERR_OVERFLOW, "tr_batch_result_panels: a row exceeds the panel capacity");
+            if (rp[r + 1] - start > max_panel_nnz) {
+                rb.push_back(r);
+                start = rp[r];
+            }
+        }
+        rb.push_back(m);
+        const int32_t np = (int32_t)rb.size() - 1;
+        *npanels = np;
+        if (np > max_panels) fail(TR_ERR_OVERFLOW, "tr_batch_result_panels: needs " + std::to_string(np) + " panels");
+        for (int32_t p = 0; p < np; ++p) {
+            const int32_t r0 = rb[p], r1 = rb[p + 1];
+            // int32 row pointers of the panel, rebased to its first entry
+            // (batch-owned: valid until tr_batch_destroy)
+            int32_t *ptr = (int32_t *)b->own(sizeof(int32_t) * ((size_t)(r1 - r0) + 1));
+            if (b->no_accum) {
+                const int32_t g0 = b->mp.h_tasks[t].row_base;
+                launch_rowptr_i64_to_i32(ctx, b->mp.rowptr, g0 + r0, g0 + r1, ptr);
+            } else {
+                std::vector<int32_t> h((size_t)(r1 - r0) + 1);
+                for (int32_t r = r0; r <= r1; ++r) h[r - r0] = (int32_t)(rp[r] - rp[r0]);
+                TR_HIP(hipMemcpyAsync(ptr, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+                TR_HIP(hipStreamSynchronize(ctx->stream));
+            }
+            tr_csx_t c{};
+            c.majordim = TR_ROWS;
+            c.nrows = r1 - r0;
+            c.ncols = b->B[t].ncols;
+            c.nnz = rp[r1] - rp[r0];
+            c.ptr_vec = ptr;
+            c.idx_vec = b->o_idx + rp[r0];
+            c.val = b->o_val + rp[r0];
+            panels[p] = c;
+            row_begin[p] = r0;
+        }
+        row_begin[np] = m;
+    });
+}
 
 int tr_batch_check(tr_batch_t b) {
     if (!b) return TR_ERR_INVALID;

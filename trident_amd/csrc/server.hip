@@ -460,7 +460,7 @@ int tr_server_wait(tr_server_t s, int64_t req_id, tr_csx_t *C, double *latency_m
     if (r->status != TR_OK) return r->status;
     if (C) {
         if (!r->result) return TR_ERR_INVALID;  // submitted with TR_REQ_DISCARD
-        tr_batch_result(r->result->b, r->result_task, C);
+        return tr_batch_result(r->result->b, r->result_task, C);
     }
     return TR_OK;
 }

@@ -380,6 +380,21 @@ class Batch:
         self.ctx.check(self.ctx.lib.tr_batch_result(self.h, t, ctypes.byref(d)), "tr_batch_result")
         return DeviceCsx(self.ctx, d)
 
+    def result_panels(self, t: int, max_panel_nnz: int = 2**31 - 1,
+                      max_panels: int = 256) -> Tuple[List[DeviceCsx], List[int]]:
+        """Task t's result as int32 row panels (views valid until close)."""
+        arr = (tr_csx_t * max_panels)()
+        rb = (ctypes.c_int32 * (max_panels + 1))()
+        npan = ctypes.c_int32()
+        self.ctx.check(self.ctx.lib.tr_batch_result_panels(self.h, t, max_panel_nnz, arr, rb, max_panels,
+                                                           ctypes.byref(npan)), "tr_batch_result_panels")
+        out = []
+        for p in range(npan.value):
+            d = tr_csx_t()
+            ctypes.memmove(ctypes.byref(d), ctypes.byref(arr[p]), ctypes.sizeof(tr_csx_t))
+            out.append(DeviceCsx(self.ctx, d))
+        return out, [int(rb[p]) for p in range(npan.value + 1)]
+
     def stats(self) -> Tuple[int, int, int]:
         f, p, o = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         self.ctx.lib.tr_batch_stats(self.h, ctypes.byref(f), ctypes.byref(p), ctypes.byref(o))
