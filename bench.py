@@ -16,6 +16,8 @@ plain vs batched/served with its dominant kernel's roofline and an oracle
 check on sampled rows:
   batched   one SUMMA stage of R-MAT scale 22 cut 4x4 (stage 3), 16 tile
             products added into partial-C tiles, one hipGraph launch;
+  batched_first_stage  stage 2 of the same grid for ranks that start there
+            (no partial C; tile products beyond int32 read as row panels);
   galerkin  P^T A P on the 27-point Laplacian 128^3 with its SA prolongator;
   served    2000 mixed requests at 1x and 4x the plain rate, p50/p99.
 
@@ -178,22 +180,28 @@ def cpu_c1_baseline(oracle, repeats=10):
             "ms": round(1000 * t, 4), "GFLOP/s": round(2 * mults / t / 1e9, 3)}
 
 
-def cpu_child(path):
-    """CPU-baseline child (bench.py --cpu-child PATH): reads the sampled rows
-    of A, B = A and the GPU's rows of C from PATH.npz, times the oracle on
-    them and on the Erdos-Renyi n=1024 product, checks the GPU rows, and
-    writes PATH.json."""
+def cpu_child():
+    """CPU-baseline child (bench.py --cpu-child).  It is started before the
+    parent touches the GPU (a process that has initialised the GPU does not
+    start other programs) and waits for one line on stdin: PATH, whose
+    PATH.npz holds the sampled rows of A, B = A and the GPU's rows of C.  It
+    times the oracle on them and on the Erdos-Renyi n=1024 product, checks
+    the GPU rows, and writes PATH.json; an empty line ends it."""
+    path = sys.stdin.readline().strip()
+    if not path:
+        return 0
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from trident_amd import gen
-    z = np.load(path + ".npz")
+    with np.load(path + ".npz") as npz:
+        z = {k: npz[k] for k in npz.files}  # each item access of an NpzFile reads the array again
     S = gen.from_arrays(int(z["s_n"]), int(z["a_n"]), z["s_ptr"], z["s_idx"], z["s_val"])
     A = gen.from_arrays(int(z["a_n"]), int(z["a_n"]), z["a_ptr"], z["a_idx"], z["a_val"])
     t0 = time.perf_counter()
     cp, ci, cv = oracle.spgemm(S, A)
     t_cpu = time.perf_counter() - t0
-    gp = z["g_ptr"]
-    got = [(z["g_idx"][gp[i]:gp[i + 1]], z["g_val"][gp[i]:gp[i + 1]]) for i in range(len(gp) - 1)]
+    gp, gi, gv = z["g_ptr"], z["g_idx"], z["g_val"]
+    got = [(gi[gp[i]:gp[i + 1]], gv[gp[i]:gp[i + 1]]) for i in range(len(gp) - 1)]
     same, max_rel = compare_rows(got, cp, ci, cv)
     out = {"t_cpu": t_cpu, "outputs": int(cp[-1]), "threads": oracle.num_threads(), "structure_equal": same,
            "max_rel_err": max_rel, "c1": cpu_c1_baseline(oracle)}
@@ -202,11 +210,21 @@ def cpu_child(path):
     return 0
 
 
-def cpu_baseline_and_check(A, f, ctx, panels, rb, hptrs, target_mults, seed=7):
+def start_cpu_child():
+    """The CPU-baseline child process (see cpu_child), with OpenMP threads =
+    the physical cores, bound close; call before any GPU work."""
+    env, phys, logical = cpu_env()
+    proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-child"], env=env,
+                            stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    return proc, env, phys, logical
+
+
+def cpu_baseline_and_check(child, A, f, ctx, panels, rb, hptrs, target_mults, seed=7):
     """Time the oracle on a random row sample of ~target_mults multiplications
-    (in a child process with OpenMP bound to the physical cores) and compare
+    (in the child process, OpenMP bound to the physical cores) and compare
     those rows of C with the GPU result."""
     import tempfile
+    proc, env, phys, logical = child
     rng = np.random.default_rng(seed)
     order = rng.permutation(A.nrows)
     cum = np.cumsum(f[order])
@@ -217,17 +235,15 @@ def cpu_baseline_and_check(A, f, ctx, panels, rb, hptrs, target_mults, seed=7):
     got = download_rows(ctx, panels, rb, hptrs, rows)
     g_ptr = np.zeros(len(got) + 1, np.int64)
     np.cumsum([g[0].shape[0] for g in got], out=g_ptr[1:])
-    env, phys, logical = cpu_env()
     with tempfile.TemporaryDirectory() as td:
         base = os.path.join(td, "cpu")
         np.savez(base + ".npz", s_n=S.nrows, s_ptr=S.ptr, s_idx=S.idx, s_val=S.val, a_n=A.nrows, a_ptr=A.ptr,
                  a_idx=A.idx, a_val=A.val, g_ptr=g_ptr,
                  g_idx=np.concatenate([g[0] for g in got]) if got else np.zeros(0, np.int32),
                  g_val=np.concatenate([g[1] for g in got]) if got else np.zeros(0))
-        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", base], env=env,
-                           capture_output=True, text=True, timeout=600)
-        if r.returncode != 0:
-            raise RuntimeError(f"CPU baseline child failed: {r.stderr[-2000:]}")
+        _, err = proc.communicate(base + "\n", timeout=600)
+        if proc.returncode != 0 or not os.path.exists(base + ".json"):
+            raise RuntimeError(f"CPU baseline child failed (exit {proc.returncode}): {err[-2000:]}")
         with open(base + ".json") as fh:
             res = json.load(fh)
     t_cpu = res["t_cpu"]
@@ -493,6 +509,134 @@ def stage_section(scale, k, steps, tuning="auto", with_compress=False, g=4):
     for c in Cin:
         if c.initialized:
             c.free()
+    ctx.close()
+    return res
+
+
+def first_stage_section(scale, k, steps, tuning="auto", g=4):
+    """Batched first stage.  SUMMA stage k of R-MAT scale S (edge factor 16,
+    seed 2) cut g x g, for ranks that start at that stage (no partial C yet):
+    the 16 tile products alone.  At scale 22, stages 1 and 2 hold tile
+    products of up to 2.9e9 entries, beyond an int32 tile, so both forms
+    return them as int32 row panels: plain, 16 tr_spgemm_panels calls;
+    batched, one hipGraph launch read with tr_batch_result_panels.  Checked
+    against the oracle on sampled rows of three tiles."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from trident_amd import gen
+    from trident_amd.device import Batch, Context, spgemm_flops, spgemm_panels
+    tun, tsrc = use_tuning(f"stage:{scale}:{k}", tuning)
+    t0 = time.perf_counter()
+    A = gen.rmat(scale, 16, seed=2)
+    n = A.nrows // g
+    need = {(i, k) for i in range(g)} | {(k, j) for j in range(g)}
+    T = {key: gen.submatrix(A, key[0] * n, (key[0] + 1) * n, key[1] * n, (key[1] + 1) * n) for key in need}
+    del A
+    log(f"[first stage] R-MAT s{scale} tiles ready in {time.perf_counter() - t0:.1f}s")
+    ctx = Context(0)
+    dT = {key: ctx.upload(m) for key, m in T.items()}
+    keys = [(i, j) for i in range(g) for j in range(g)]
+    fl = [spgemm_flops(dT[(i, k)], dT[(k, j)]) for i, j in keys]
+    mults = sum(fl)
+
+    def plain_step():
+        outs = []
+        for i, j in keys:
+            panels, _ = spgemm_panels(dT[(i, k)], dT[(k, j)], max_panels=64)
+            outs.extend(panels)
+        ctx.synchronize()
+        return outs
+
+    for P in plain_step():
+        P.free()
+    ts = []
+    nnz_out = 0
+    for _ in range(steps):
+        t = time.perf_counter()
+        outs = plain_step()
+        ts.append(time.perf_counter() - t)
+        nnz_out = sum(P.nnz for P in outs)
+        for P in outs:
+            P.free()
+    t_plain = float(np.median(ts))
+    res = {"plain": {"ms": round(1000 * t_plain, 2), "GFLOP/s": round(2 * mults / t_plain / 1e9, 2)}}
+    log(f"[first stage] plain: {res['plain']}")
+    b = Batch(ctx, [dT[(i, k)] for i, j in keys], [dT[(k, j)] for i, j in keys])
+    ts = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        b.run()
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t)
+    b.check()
+    t_b = float(np.median(ts))
+    res["batched"] = {"ms": round(1000 * t_b, 2), "GFLOP/s": round(2 * mults / t_b / 1e9, 2)}
+    res["speedup_batched_vs_plain"] = round(t_plain / t_b, 3)
+    log(f"[first stage] batched: {res['batched']}")
+    # oracle on sampled rows of three tiles (the largest product among them)
+    rng = np.random.default_rng(12)
+    order = [t for t in np.argsort(fl) if fl[t] > 0]
+    picks = sorted({int(order[-1]), int(order[0]), int(order[len(order) // 2])})
+    ok, max_rel, checked, big = True, 0.0, 0, 0
+    per_row_out = {}
+    for t in range(len(keys)):
+        panels, rb = b.result_panels(t, max_panels=64)
+        big = max(big, len(panels))
+        hp = np.concatenate([[0]] + [np.diff(ctx.to_host(P.d.ptr_vec, (P.d.nrows + 1,), np.int32).astype(np.int64))
+                                     for P in panels])
+        per_row_out[t] = hp[1:]
+        if t not in picks:
+            continue
+        i, j = keys[t]
+        rows = np.sort(rng.choice(T[(i, k)].nrows, 48, replace=False))
+        got = []
+        for r in rows:
+            p = int(np.searchsorted(rb, r, side="right") - 1)
+            got.extend(device_rows(ctx, panels[p], [r - rb[p]]))
+        same, rel = compare_rows(got, *oracle.spgemm(take_rows(T[(i, k)], rows), T[(k, j)]))
+        ok &= same
+        max_rel = max(max_rel, rel)
+        checked += len(rows)
+    res["oracle_sample"] = {"tiles": [list(keys[t]) for t in picks], "rows_checked": checked,
+                            "structure_equal": bool(ok), "max_rel_err": max_rel}
+    res["max_panels_per_tile"] = big
+    log(f"[first stage] oracle sample: {res['oracle_sample']}, up to {big} panels per tile")
+    b.close()
+    # per-kernel HIP-event times of one eager pass, and the dominant kernel's
+    # algorithmic bytes from each tile's row classes
+    e = Batch(ctx, [dT[(i, k)] for i, j in keys], [dT[(k, j)] for i, j in keys], graph=False)
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    e.run()
+    ctx.synchronize()
+    ctx.set_timing(False)
+    kt = ctx.timings()
+    e.close()
+    lb = hb = 0
+    for t, (i, j) in enumerate(keys):
+        At, Bt = T[(i, k)], T[(k, j)]
+        blen = np.diff(Bt.ptr.astype(np.int64))
+        f = np.zeros(At.nrows, np.int64)
+        np.add.at(f, At.row_ids(), blen[At.idx])
+        l_, h_ = class_bytes(np.diff(At.ptr.astype(np.int64)), f, per_row_out[t], 12 * Bt.nnz + 4 * (Bt.nrows + 1),
+                             tun["light_max_flops"])
+        lb += l_
+        hb += h_
+
+    def alg(name, launches):
+        if name == "num_heavy":
+            return (hb / max(launches, 1), "heavy rows of the 16 products: 12 B per C entry written and per A "
+                                           "entry read + B read once per tile")
+        if name == "num_light":
+            return (lb / max(launches, 1), "light rows of the 16 products: 12 B per C entry written and per A "
+                                           "entry read + B read once per tile")
+        return None
+    res["kernels_ms"] = {kk: round(v[1], 3) for kk, v in sorted(kt.items())}
+    res["roofline"] = kernel_roofline(kt, 1, alg)
+    res.update({"workload": f"R-MAT scale {scale} (ef 16, seed 2) cut {g}x{g}; SUMMA stage k={k} of ranks that "
+                            f"start there: {g * g} tile products, no partial C, results as int32 row panels",
+                "mults": mults, "nnz_out": nnz_out, "tuning": tun, "tuning_source": tsrc})
     ctx.close()
     return res
 
@@ -827,15 +971,17 @@ def main():
     ap.add_argument("--cpu-mults", type=float, default=1.5e9,
                     help="multiplications in the CPU-baseline row sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-child", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
     ap.add_argument("--serve-workers", type=int, default=1,
                     help="server worker threads (one batch at a time keeps the GPU's caches for one product)")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[1.0, 4.0])
     ap.add_argument("--sections", default="all",
                     help="forms measured after the plain product on its default workload: all, or none, or a "
-                         "comma list of batched,galerkin,served")
+                         "comma list of batched,first_stage,galerkin,served")
     ap.add_argument("--stage", default="22:3", help="batched form: R-MAT scale and SUMMA stage, S:k")
+    ap.add_argument("--first-stage", default="22:2",
+                    help="batched first stage (no partial C, results beyond int32 as panels): S:k")
     ap.add_argument("--galerkin", type=int, default=128, help="Galerkin form: Laplacian edge n (n^3 grid)")
     ap.add_argument("--serve-requests", type=int, default=2000, help="served form: requests in the stream")
     ap.add_argument("--tuning", default="auto",
@@ -846,7 +992,11 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         return 2
     if args.cpu_child:
-        return cpu_child(args.cpu_child)
+        return cpu_child()
+    # the plain product's CPU-baseline child starts before this process loads
+    # the HIP library or touches the GPU
+    plain = not args.workload.startswith(("copy:", "stage:", "galerkin:", "serve"))
+    child = start_cpu_child() if plain and not args.no_cpu else None
     if args.workload.startswith("copy:"):
         return copy_bench(args)
     args.tuning_used, args.tuning_source = apply_tuning(args)
@@ -958,7 +1108,7 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline + sample check ...")
         t = time.perf_counter()
-        cpu, check = cpu_baseline_and_check(A, f, ctx, panels, rb, hptrs, args.cpu_mults)
+        cpu, check = cpu_baseline_and_check(child, A, f, ctx, panels, rb, hptrs, args.cpu_mults)
         log(f"[bench] cpu baseline done in {time.perf_counter() - t:.1f}s: {cpu['value']} GFLOP/s, check {check}")
 
     out = {
@@ -997,12 +1147,15 @@ def main():
     # the batched, Galerkin and served forms, each with its dominant kernel's
     # roofline and an oracle check on sampled rows
     want = set() if args.sections == "none" else (
-        {"batched", "galerkin", "served"} if args.sections == "all" else set(args.sections.split(",")))
+        {"batched", "first_stage", "galerkin", "served"} if args.sections == "all" else set(args.sections.split(",")))
     if args.workload != "rmat:20":
         want = set() if args.sections == "all" else want
     if "batched" in want:
         sc, kk = (int(x) for x in args.stage.split(":"))
         out["batched"] = stage_section(sc, kk, args.steps, args.tuning)
+    if "first_stage" in want:
+        sc, kk = (int(x) for x in args.first_stage.split(":"))
+        out["batched_first_stage"] = first_stage_section(sc, kk, args.steps, args.tuning)
     if "galerkin" in want:
         out["galerkin"] = galerkin_section(args.galerkin, args.steps, args.tuning)
     if "served" in want:
