@@ -1158,6 +1158,33 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
                     // this step in thread (= A) order: the list is gathered into a
                     // bitmask of thread ids and walked from the lowest bit
                     if (act && (L.keytag[h] & kNoTag) == (uint32_t)tid) {
+                      // lists of up to three products are ordered by compare
+                      // and swap; longer ones through a bitmask of ids
+                      const uint32_t t1 = L.link[tid];
+                      const uint32_t t2 = t1 == kNoTag ? kNoTag : (uint32_t)L.link[t1];
+                      const uint32_t t3 = t2 == kNoTag ? kNoTag : (uint32_t)L.link[t2];
+                      if (t3 == kNoTag) {
+                        double v = L.vals[h];
+                        if (t1 == kNoTag) {
+                            v = __fma_rn(ao, b, v);
+                        } else {
+                            uint32_t u0 = (uint32_t)tid, u1 = t1, u2 = t2;  // t2 == kNoTag sorts last
+                            uint32_t lo = min(u0, u1), hi = max(u0, u1);
+                            u0 = lo;
+                            u1 = hi;
+                            lo = min(u1, u2), hi = max(u1, u2);
+                            u1 = lo;
+                            u2 = hi;
+                            lo = min(u0, u1), hi = max(u0, u1);
+                            u0 = lo;
+                            u1 = hi;
+                            v = __fma_rn(L.pa[u0], L.pb[u0], v);
+                            v = __fma_rn(L.pa[u1], L.pb[u1], v);
+                            if (u2 != kNoTag) v = __fma_rn(L.pa[u2], L.pb[u2], v);
+                        }
+                        L.vals[h] = v;
+                        L.keytag[h] = key | kNoTag;
+                      } else {
                         uint64_t mk[NT / kWave];
 #pragma unroll
                         for (int w = 0; w < NT / kWave; ++w) mk[w] = 0ull;
@@ -1178,6 +1205,7 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
                         }
                         L.vals[h] = v;
                         L.keytag[h] = key | kNoTag;
+                      }
                     }
                     if (x0 + 2 * NT < total) mark_segment<NT>(L.smask[bC], L.smark[bC], x0 + 2 * NT, excl, slen);
                     __syncthreads();
