@@ -254,7 +254,8 @@ def test_heavy_items_beyond_grid_cap(ctx):
     # numeric grid (one workgroup per item, and a dispatch counts its
     # work-items in 32 bits), so workgroups stride over the items.  Every
     # cutover keeps the result bit-identical: it must equal the default
-    # tuning's panels.
+    # tuning's panels, and the oracle on sampled rows (the 16 heaviest
+    # among them).
     A = gen.rmat(19, 16, seed=1)
     dA = ctx.upload(A)
     nnz, _ = symbolic(dA, dA)
@@ -272,6 +273,24 @@ def test_heavy_items_beyond_grid_cap(ctx):
         assert _device_bytes_equal(ctx, p.d.ptr_vec, q.d.ptr_vec, 4 * (p.d.nrows + 1)), "row pointers differ"
         assert _device_bytes_equal(ctx, p.d.idx_vec, q.d.idx_vec, 4 * p.nnz), "column indices differ"
         assert _device_bytes_equal(ctx, p.d.val, q.d.val, 8 * p.nnz), "values differ"
+    # and the oracle agrees on sampled rows (heavy ones included)
+    rng = np.random.default_rng(19)
+    heavy = np.argsort(np.diff(A.ptr.astype(np.int64)))[-16:]
+    rows = np.unique(np.concatenate([heavy, rng.choice(A.nrows, 240, replace=False)]))
+    lens = np.diff(A.ptr.astype(np.int64))[rows]
+    sp = np.zeros(len(rows) + 1, np.int64)
+    np.cumsum(lens, out=sp[1:])
+    sel = np.concatenate([np.arange(A.ptr[r], A.ptr[r + 1]) for r in rows])
+    cp, ci, cv = oracle.spgemm(gen.from_arrays(len(rows), A.ncols, sp, A.idx[sel], A.val[sel]), A)
+    for t, r in enumerate(rows):
+        pn = int(np.searchsorted(rb_got, r, side="right") - 1)
+        P = got[pn]
+        pp = P.ctx.to_host(P.d.ptr_vec + 4 * int(r - rb_got[pn]), (2,), np.int32)
+        s0, e0 = int(pp[0]), int(pp[1])
+        gi = P.ctx.to_host(P.d.idx_vec + 4 * s0, (e0 - s0,), np.int32)
+        gv = P.ctx.to_host(P.d.val + 8 * s0, (e0 - s0,), np.float64)
+        assert np.array_equal(gi, ci[cp[t]:cp[t + 1]]), f"row {r}: columns differ from the oracle"
+        assert np.array_equal(gv.view(np.uint64), cv[cp[t]:cp[t + 1]].view(np.uint64)), f"row {r}: values differ"
     for p in got + ref:
         p.free()
     dA.free()
