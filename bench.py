@@ -936,21 +936,39 @@ def serve_bench(args):
 
 
 def copy_bench(args):
-    """Plain device copy of a known size (an elementwise y = x * 1 over fp64):
-    the calibration run for the FETCH_SIZE / WRITE_SIZE counters, whose
-    readings for this kernel are compared with the bytes it must move."""
-    import torch
+    """Plain device copy of a known size: the calibration run for the
+    FETCH_SIZE / WRITE_SIZE counters, whose readings for the copy kernel are
+    compared with the bytes it must move.  copy:G uses torch's elementwise
+    y = x * 1 over fp64; hipcopy:G this library's own streaming copy kernel
+    (tr_calibration_copy), a second, independent calibration."""
     gib = float(args.workload.split(":")[1])
-    n = int(gib * (1 << 30)) // 8
-    x = torch.rand(n, dtype=torch.float64, device="cuda:0")
-    y = torch.empty_like(x)
+    if args.workload.startswith("hipcopy:"):
+        from trident_amd.device import Context
+        nbytes = (int(gib * (1 << 30)) // 16) * 16
+        ctx = Context(0)
+        src, dst = ctx.malloc(nbytes), ctx.malloc(nbytes)
+        ctx.check(ctx.lib.tr_memset(ctx.h, ctypes.c_void_p(src), 1, nbytes), "memset")
+
+        def run():
+            ctx.check(ctx.lib.tr_calibration_copy(ctx.h, ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes), "copy")
+        sync = ctx.synchronize
+        n = nbytes // 8
+    else:
+        import torch
+        n = int(gib * (1 << 30)) // 8
+        x = torch.rand(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+
+        def run():
+            torch.mul(x, 1.0, out=y)
+        sync = torch.cuda.synchronize
     for _ in range(max(args.warmup, 1)):
-        torch.mul(x, 1.0, out=y)
-    torch.cuda.synchronize()
+        run()
+    sync()
     t = time.perf_counter()
     for _ in range(args.steps):
-        torch.mul(x, 1.0, out=y)
-    torch.cuda.synchronize()
+        run()
+    sync()
     dt = (time.perf_counter() - t) / max(args.steps, 1)
     nbytes = 8 * n
     out = {"metric": "device copy GB/s", "value": round(2 * nbytes / dt / 1e9, 2), "unit": "GB/s", "n_gpus": 1,
@@ -995,9 +1013,9 @@ def main():
         return cpu_child()
     # the plain product's CPU-baseline child starts before this process loads
     # the HIP library or touches the GPU
-    plain = not args.workload.startswith(("copy:", "stage:", "galerkin:", "serve"))
+    plain = not args.workload.startswith(("copy:", "hipcopy:", "stage:", "galerkin:", "serve"))
     child = start_cpu_child() if plain and not args.no_cpu else None
-    if args.workload.startswith("copy:"):
+    if args.workload.startswith(("copy:", "hipcopy:")):
         return copy_bench(args)
     args.tuning_used, args.tuning_source = apply_tuning(args)
     if args.workload.startswith("stage:"):

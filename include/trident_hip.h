@@ -379,6 +379,13 @@ int tr_ctx_timing_count(tr_ctx_t ctx);
  * total milliseconds.  Synchronises the stream. */
 int tr_ctx_timing_get(tr_ctx_t ctx, int i, char *name, int name_len, int64_t *launches, double *total_ms);
 
+/* dst = src with a plain streaming copy kernel of this library (16 bytes per
+ * lane, grid-stride): the reference workload for calibrating hardware
+ * traffic counters (FETCH_SIZE / WRITE_SIZE) against bytes known to move.
+ * Pointers 16-byte aligned, bytes a multiple of 16.  No counterpart in the
+ * reference. */
+int tr_calibration_copy(tr_ctx_t ctx, void *dst, const void *src, size_t bytes);
+
 /* Library build string (architecture and tuning constants). */
 const char *tr_version(void);
 

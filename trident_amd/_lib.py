@@ -130,6 +130,7 @@ SIGNATURES = [
     ("tr_ctx_timing_count", ctypes.c_int, [_vp]),
     ("tr_ctx_timing_get", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _P(_i64),
                                          _P(ctypes.c_double)]),
+    ("tr_calibration_copy", ctypes.c_int, [_vp, _vp, _vp, _sz]),
     ("tr_version", ctypes.c_char_p, []),
 ]
 

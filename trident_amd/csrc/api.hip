@@ -491,6 +491,16 @@ int tr_spadd_into(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *
     });
 }
 
+int tr_calibration_copy(tr_ctx_t ctx, void *dst, const void *src, size_t bytes) {
+    TR_CTX_REQUIRED(ctx);
+    return guarded(ctx, [&] {
+        TR_REQUIRE(bytes % 16 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0,
+                   "tr_calibration_copy: 16-byte aligned pointers and a multiple of 16 bytes");
+        TR_REQUIRE(bytes == 0 || (dst && src), "tr_calibration_copy: null pointer");
+        launch_copy16(ctx, dst, src, bytes);
+    });
+}
+
 int tr_spadd_into_noalloc(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, tr_csx_t *out, int64_t capacity,
                           int64_t *nnz_needed) {
     TR_CTX_REQUIRED(ctx);

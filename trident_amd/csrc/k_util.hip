@@ -94,6 +94,12 @@ void scan_impl(tr_ctx *ctx, const Tin *in, int64_t *out, int64_t n, int64_t *tmp
     TR_LAUNCH(ctx, "scan_apply", k_scan_apply<Tin>, (uint32_t)ntiles, kScanThreads, 0, in, n, sums, out);
 }
 
+// plain streaming copy, 16 bytes per lane per iteration (counter calibration)
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n16) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -279,6 +285,13 @@ void launch_exclusive_scan_i64(tr_ctx *ctx, const int64_t *in, int64_t *out, int
 }
 void launch_exclusive_scan_i32_to_i64(tr_ctx *ctx, const int32_t *in, int64_t *out, int64_t n, int64_t *tmp) {
     scan_impl<int32_t>(ctx, in, out, n, tmp);
+}
+
+void launch_copy16(tr_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    const int64_t n16 = (int64_t)(bytes / 16);
+    if (n16 <= 0) return;
+    const uint32_t g = (uint32_t)std::min<int64_t>((n16 + 255) / 256, (int64_t)ctx->cu_count * 32);
+    TR_LAUNCH(ctx, "copy16", k_copy16, g, 256, 0, (const uint4 *)src, (uint4 *)dst, n16);
 }
 
 void launch_fill_i32(tr_ctx *ctx, int32_t *p, int32_t v, int64_t n) {

@@ -196,6 +196,7 @@ void launch_exclusive_scan_i32_to_i64(tr_ctx *ctx, const int32_t *in, int64_t *o
 size_t validate_tmp_bytes(int64_t nnz);
 void launch_validate(tr_ctx *ctx, const tr_csx_t *m, int32_t *d_status, uint32_t *starts = nullptr);
 void launch_fill_i32(tr_ctx *ctx, int32_t *p, int32_t v, int64_t n);
+void launch_copy16(tr_ctx *ctx, void *dst, const void *src, size_t bytes);
 void launch_transpose(tr_ctx *ctx, const tr_csx_t *m, tr_csx_t *out);
 void launch_sort_rows(tr_ctx *ctx, tr_csx_t *m);
 void launch_rowptr_i64_to_i32(tr_ctx *ctx, const int64_t *rp64, int32_t r0, int32_t r1, int32_t *out);
