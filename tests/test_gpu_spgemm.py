@@ -364,3 +364,24 @@ def test_invalid_operands_rejected(ctx):
     with pytest.raises(TridentError) as e:
         spgemm(dbad, dA)
     assert e.value.code == TR_ERR_INVALID
+
+
+@pytest.mark.parametrize("width", [24, 60, 120, 136, 400])
+def test_heavy_rows_with_few_entries(ctx, width):
+    # Rows with thousands of multiplications but few distinct columns: a
+    # one-pass heavy row with <= 128 entries goes to the light numeric (table
+    # of twice its entries, lane-mask ordering), wider ones keep windows.
+    # Every B row draws 40 columns from a band of `width`, so each output sums
+    # about 40 * 96 / width products in A-order.
+    rng = np.random.default_rng(width)
+    n, band = 4096, width
+    def rows_from(cols_per_row, lo, hi, nrows):
+        idx = [np.sort(rng.choice(np.arange(lo, hi), size=cols_per_row, replace=False)) for _ in range(nrows)]
+        ptr = np.arange(0, (nrows + 1) * cols_per_row, cols_per_row, dtype=np.int64)
+        return ptr, np.concatenate(idx).astype(np.int32)
+    ap, ai = rows_from(96, 0, n, 256)
+    A = gen.from_arrays(256, n, ap, ai, rng.uniform(-1, 1, ai.size))
+    bp, bi = rows_from(min(40, band), 1000, 1000 + band, n)
+    B = gen.from_arrays(n, n, bp, bi, rng.uniform(-1, 1, bi.size))
+    print(f"few-entry heavy rows: band {width}", flush=True)
+    run(ctx, A, B, f"band {width}")

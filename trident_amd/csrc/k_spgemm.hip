@@ -366,7 +366,7 @@ template <int PB, int NTH>
 __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 1) void k_sym_heavy(
     const TaskDev *__restrict__ tasks, int nt, const int32_t *__restrict__ rows, Counters *__restrict__ cnt, int cls,
     int64_t *__restrict__ rownnz, int emit, int32_t win_h, int32_t item_windows, Window *__restrict__ wins,
-    int32_t win_cap, Item *__restrict__ items, int32_t item_cap) {
+    int32_t win_cap, Item *__restrict__ items, int32_t item_cap, int32_t *__restrict__ lists, int32_t rows_total) {
     constexpr int kWords = 1 << (PB - 5);          // bitmap words of one pass
     constexpr int kFine = 1 << (PB - kFineShift);  // fine blocks of one pass
     constexpr int kPerThread = kFine / NTH;        // consecutive fine blocks per thread: 2..8
@@ -457,6 +457,21 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
                 }
             }
             if (pass_total == 0 || !emit) {
+                row_total += pass_total;
+                __syncthreads();
+                continue;
+            }
+            // A row done in one pass whose product has few entries (many
+            // multiplications per entry: the coarse Galerkin product has about
+            // 55) goes to the light numeric, whose small table and lane-mask
+            // ordering suit it, instead of to windows: appended to the light
+            // class whose table holds twice its entries.
+            if (ncols <= (1 << PB) && pass_total <= kCompactMax) {
+                if (tid == 0) {
+                    const int c = pass_total <= 32 ? 0 : pass_total <= 64 ? 1 : 2;
+                    const int32_t pos = atomicAdd(&cnt->cls[c], 1);
+                    lists[(int64_t)c * rows_total + pos] = g;
+                }
                 row_total += pass_total;
                 __syncthreads();
                 continue;
@@ -1461,7 +1476,8 @@ template <int PB, int NTH>
 void launch_sym_heavy(tr_ctx *ctx, MulPlan &pl, int cls, int per_cu, bool emit) {
     TR_LAUNCH(ctx, "sym_heavy", (k_sym_heavy<PB, NTH>), cap_grid(pl.h.cls[cls], (int64_t)ctx->cu_count * per_cu), NTH,
               0, pl.d_tasks, pl.nt, pl.lists + (int64_t)cls * pl.rows_total, pl.cnt, cls, pl.rownnz, emit ? 1 : 0,
-              pl.tune.window_entries, pl.tune.item_windows, pl.wins, pl.win_cap, pl.items, pl.item_cap);
+              pl.tune.window_entries, pl.tune.item_windows, pl.wins, pl.win_cap, pl.items, pl.item_cap, pl.lists,
+              pl.rows_total);
 }
 
 void mul_symbolic(tr_ctx *ctx, Workspace &ws, MulPlan &pl, bool emit, bool capture) {

@@ -31,6 +31,8 @@ constexpr int kSparseAvg = 192;    // a pass with <= this many entries per width
 constexpr int kSparseH = 256;      // sparse windows: count quantum (<= kSparseH + 127 entries, any width)
 constexpr int kHeavyThreads = 1024;  // symbolic heavy block
 constexpr int kLongRow = 2048;       // heavy rows with more A entries take shorter items
+constexpr int kCompactMax = 128;     // one-pass heavy rows with <= this many entries take the light numeric
+static_assert(kCompactMax * 2 <= (64 << 2), "compact rows fit the light tables of classes 0..2");
 
 struct Window {
     int32_t c0, c1;   // column range [c0, c1)
