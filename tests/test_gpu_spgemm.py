@@ -232,6 +232,46 @@ def test_light_short_rows_ordered(ctx, per_row):
     run(ctx, A, B, "short B rows")
 
 
+def test_heavy_ordered_collisions(ctx):
+    # Heavy rows of 1500 A entries over B rows that all hold four hub
+    # columns, a few of 32 semi-hub columns and random ones: each 256-product
+    # step of a row's first window meets every hub slot about 17 times and
+    # the semi-hub slots once to three times, so the step's slot lists take
+    # every apply path (one, two, three and longer products).  Values span
+    # 2^-20..2^20.  The same products summed in the reverse A-order (A's
+    # columns relabelled k -> K-1-k, B's rows reversed to match) differ in
+    # most hub and semi-hub entries, so the bit-exact match pins the order.
+    rng = np.random.default_rng(29)
+    k, ncols, m = 3000, 40000, 48
+    bl = []
+    for _ in range(k):
+        semi = np.nonzero(rng.random(32) < 0.1)[0] + 4
+        rand = rng.choice(np.arange(36, ncols), size=8, replace=False)
+        bl.append(np.sort(np.concatenate([np.arange(4), semi, rand])))
+    bptr = np.concatenate([[0], np.cumsum([len(x) for x in bl])])
+    bval = rng.uniform(-1, 1, bptr[-1]) * np.exp2(rng.integers(-20, 21, bptr[-1]))
+    B = gen.from_arrays(k, ncols, bptr, np.concatenate(bl), bval)
+    al = [np.sort(rng.choice(k, size=1500, replace=False)) for _ in range(m)]
+    aptr = np.concatenate([[0], np.cumsum([len(x) for x in al])])
+    aidx = np.concatenate(al)
+    aval = rng.uniform(-1, 1, aptr[-1]) * np.exp2(rng.integers(-20, 21, aptr[-1]))
+    A = gen.from_arrays(m, k, aptr, aidx, aval)
+    C = run(ctx, A, B, "heavy ordered collisions")
+    ridx = np.concatenate([(k - 1 - aidx[aptr[i]:aptr[i + 1]])[::-1] for i in range(m)])
+    rval = np.concatenate([aval[aptr[i]:aptr[i + 1]][::-1] for i in range(m)])
+    Ar = gen.from_arrays(m, k, aptr, ridx, rval)
+    rl = bl[::-1]
+    rptr = np.concatenate([[0], np.cumsum([len(x) for x in rl])])
+    rbval = np.concatenate([bval[bptr[i]:bptr[i + 1]] for i in range(k - 1, -1, -1)])
+    Br = gen.from_arrays(k, ncols, rptr, np.concatenate(rl), rbval)
+    rp, ri, rv = oracle.spgemm(Ar, Br)
+    assert np.array_equal(rp, C.ptr.astype(np.int64)) and np.array_equal(ri, C.idx)
+    multi = C.idx < 36  # hub and semi-hub columns: the ones with several products
+    frac = float(np.mean(rv[multi] != C.val[multi]))
+    print(f"reverse A-order changes {frac:.1%} of the hub and semi-hub entries", flush=True)
+    assert frac > 0.5
+
+
 def _device_bytes_equal(ctx, pa, pb, nbytes, chunk=1 << 29):
     """Byte equality of two device ranges, copied to the host in chunks."""
     ha = np.empty(max(1, min(chunk, nbytes)), np.uint8)

@@ -961,6 +961,30 @@ __device__ __forceinline__ int owner_marked(const unsigned long long *smask, con
     return (int)smark[w * 64 + 63 - __clzll((long long)m)];
 }
 
+// orders two products (thread id, A value, B value) by thread id
+__device__ __forceinline__ void order_products(uint32_t &u, double &a, double &b, uint32_t &w, double &c,
+                                               double &d) {
+    if (w < u) {
+        const uint32_t tu = u;
+        const double ta = a, tb = b;
+        u = w;
+        a = c;
+        b = d;
+        w = tu;
+        c = ta;
+        d = tb;
+    }
+}
+
+// Every word is written with a select, so the mask stays in registers
+// (a conditional store to one element becomes an indexed private array).
+template <int NT>
+__device__ __forceinline__ void mask_set(uint64_t (&mk)[NT / kWave], uint32_t t) {
+    const uint64_t bit = 1ull << (t & 63);
+#pragma unroll
+    for (int w = 0; w < NT / kWave; ++w) mk[w] |= ((int)(t >> 6) == w) ? bit : 0ull;
+}
+
 // workgroup exclusive scan of v; returns the exclusive prefix, *total = sum
 template <int NT>
 __device__ __forceinline__ int32_t wg_excl_scan(int32_t v, int32_t *wsum, int32_t *total) {
@@ -1128,6 +1152,16 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
                     const int32_t j = jn;
                     const double b = bn, ao = an;
                     const int32_t xn = x + NT;
+                    // claim or join this product's slot first: the CAS round
+                    // trip overlaps the next step's owner lookup below
+                    uint32_t h = 0, key = 0, me = 0, old = kSlotFree;
+                    if (act) {
+                        const uint32_t off = (uint32_t)(j - W.c0);
+                        key = off << kTagBits;
+                        me = key | (uint32_t)tid;
+                        h = hash_slot((int32_t)off, kLog);
+                        old = atomicCAS(&L.keytag[h], kSlotFree, me);
+                    }
                     if (xn < total) {
                         const int o = owner_marked(L.smask[bA], L.smark[bA], tid);
                         const int32_t q = L.seg_d[o] + xn;
@@ -1139,15 +1173,10 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
                     // insert: the first product of a slot in this step creates or
                     // claims it; every later one pushes itself onto the slot's
                     // list (the tag field holds the last pusher, link[] the rest)
-                    uint32_t h = 0, key = 0;
                     if (act) {
-                        const uint32_t off = (uint32_t)(j - W.c0);
-                        key = off << kTagBits;
-                        const uint32_t me = key | (uint32_t)tid;
-                        h = hash_slot((int32_t)off, kLog);
+                        const uint32_t off = key >> kTagBits;
                         // probe to a free slot (claimed) or this key's slot; the
                         // bound only guards a table the symbolic counts overfilled
-                        uint32_t old = atomicCAS(&L.keytag[h], kSlotFree, me);
                         for (int probe = 1; old != kSlotFree && (old >> kTagBits) != off && probe < T; ++probe) {
                             h = (h + 1) & (T - 1);
                             old = atomicCAS(&L.keytag[h], kSlotFree, me);
@@ -1170,57 +1199,65 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
                     }
                     __syncthreads();
                     // the last pusher of each slot applies the slot's products of
-                    // this step in thread (= A) order: the list is gathered into a
-                    // bitmask of thread ids and walked from the lowest bit
-                    if (act && (L.keytag[h] & kNoTag) == (uint32_t)tid) {
-                      // lists of up to three products are ordered by compare
-                      // and swap; longer ones through a bitmask of ids
-                      const uint32_t t1 = L.link[tid];
-                      const uint32_t t2 = t1 == kNoTag ? kNoTag : (uint32_t)L.link[t1];
-                      const uint32_t t3 = t2 == kNoTag ? kNoTag : (uint32_t)L.link[t2];
-                      if (t3 == kNoTag) {
+                    // this step in thread (= A) order.  The tag, the pusher's link
+                    // and the slot's value are read in one round trip, and each
+                    // list member's operands with its link; lists of up to three
+                    // are ordered in registers, longer ones walked through a
+                    // bitmask of thread ids (reading two or four operands per
+                    // round trip there measured 2% and 7% slower)
+                    if (act) {
+                        const uint32_t kt = L.keytag[h];
+                        const uint32_t t1 = L.link[tid];
                         double v = L.vals[h];
-                        if (t1 == kNoTag) {
-                            v = __fma_rn(ao, b, v);
-                        } else {
-                            uint32_t u0 = (uint32_t)tid, u1 = t1, u2 = t2;  // t2 == kNoTag sorts last
-                            uint32_t lo = min(u0, u1), hi = max(u0, u1);
-                            u0 = lo;
-                            u1 = hi;
-                            lo = min(u1, u2), hi = max(u1, u2);
-                            u1 = lo;
-                            u2 = hi;
-                            lo = min(u0, u1), hi = max(u0, u1);
-                            u0 = lo;
-                            u1 = hi;
-                            v = __fma_rn(L.pa[u0], L.pb[u0], v);
-                            v = __fma_rn(L.pa[u1], L.pb[u1], v);
-                            if (u2 != kNoTag) v = __fma_rn(L.pa[u2], L.pb[u2], v);
-                        }
-                        L.vals[h] = v;
-                        L.keytag[h] = key | kNoTag;
-                      } else {
-                        uint64_t mk[NT / kWave];
+                        if ((kt & kNoTag) == (uint32_t)tid) {
+                            if (t1 == kNoTag) {
+                                v = __fma_rn(ao, b, v);
+                            } else {
+                                const uint32_t t2 = L.link[t1];
+                                const double a1 = L.pa[t1], b1 = L.pb[t1];
+                                if (t2 == kNoTag) {
+                                    if ((uint32_t)tid < t1) {
+                                        v = __fma_rn(ao, b, v);
+                                        v = __fma_rn(a1, b1, v);
+                                    } else {
+                                        v = __fma_rn(a1, b1, v);
+                                        v = __fma_rn(ao, b, v);
+                                    }
+                                } else {
+                                    const uint32_t t3 = L.link[t2];
+                                    const double a2 = L.pa[t2], b2 = L.pb[t2];
+                                    if (t3 == kNoTag) {
+                                        uint32_t u0 = (uint32_t)tid, u1 = t1, u2 = t2;
+                                        double ea0 = ao, eb0 = b, ea1 = a1, eb1 = b1, ea2 = a2, eb2 = b2;
+                                        order_products(u0, ea0, eb0, u1, ea1, eb1);
+                                        order_products(u1, ea1, eb1, u2, ea2, eb2);
+                                        order_products(u0, ea0, eb0, u1, ea1, eb1);
+                                        v = __fma_rn(ea0, eb0, v);
+                                        v = __fma_rn(ea1, eb1, v);
+                                        v = __fma_rn(ea2, eb2, v);
+                                    } else {
+                                        uint64_t mk[NT / kWave];
 #pragma unroll
-                        for (int w = 0; w < NT / kWave; ++w) mk[w] = 0ull;
-                        for (uint32_t t = (uint32_t)tid; t != kNoTag; t = L.link[t]) {
+                                        for (int w = 0; w < NT / kWave; ++w) mk[w] = 0ull;
+                                        mask_set<NT>(mk, (uint32_t)tid);
+                                        mask_set<NT>(mk, t1);
+                                        mask_set<NT>(mk, t2);
+                                        for (uint32_t t = t3; t != kNoTag; t = L.link[t]) mask_set<NT>(mk, t);
 #pragma unroll
-                            for (int w = 0; w < NT / kWave; ++w)
-                                if ((int)(t >> 6) == w) mk[w] |= 1ull << (t & 63);
-                        }
-                        double v = L.vals[h];
-#pragma unroll
-                        for (int w = 0; w < NT / kWave; ++w) {
-                            uint64_t mw = mk[w];
-                            while (mw) {
-                                const int t = w * kWave + __ffsll((unsigned long long)mw) - 1;
-                                mw &= mw - 1ull;
-                                v = __fma_rn(L.pa[t], L.pb[t], v);
+                                        for (int w = 0; w < NT / kWave; ++w) {
+                                            uint64_t mw = mk[w];
+                                            while (mw) {
+                                                const int t = w * kWave + __ffsll((unsigned long long)mw) - 1;
+                                                mw &= mw - 1ull;
+                                                v = __fma_rn(L.pa[t], L.pb[t], v);
+                                            }
+                                        }
+                                    }
+                                }
                             }
+                            L.vals[h] = v;
+                            L.keytag[h] = key | kNoTag;
                         }
-                        L.vals[h] = v;
-                        L.keytag[h] = key | kNoTag;
-                      }
                     }
                     if (x0 + 2 * NT < total) mark_segment<NT>(L.smask[bC], L.smark[bC], x0 + 2 * NT, excl, slen);
                     __syncthreads();
