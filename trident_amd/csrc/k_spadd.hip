@@ -119,11 +119,33 @@ __device__ __forceinline__ int32_t merge_split(const TR_GLOBAL int32_t *__restri
     return lo;
 }
 
-constexpr int kSub = 256;                 // merged positions per LDS-staged sub-step
-constexpr int kPerLane = kSub / kWave;    // merged positions per lane per sub-step
-constexpr int kStage = (kSub + 1) / kWave + 1;  // tile entries (kSub + 1) per lane, rounded up
 constexpr int32_t kGuardLo = INT32_MIN;   // "no X entry before the tile"
 constexpr int32_t kGuardHi = INT32_MAX;   // "no Y entry after the row"
+
+// Setup of one chunk, loaded ahead of its use (values in lane 0)
+struct ChunkAhead {
+    int64_t xs, ys, cs, out;
+    int32_t lx, ly, g, task;
+};
+
+// Lane 0's vector loads of chunk c's row spans, chunk start and output
+// offset; the task (found through the scalar cache) and its row pointer
+// arrays are given
+template <bool kWrite>
+__device__ __forceinline__ ChunkAhead chunk_ahead(const AddTaskDev &T, int task, const int64_t *__restrict__ cstart,
+                                                  const int64_t *__restrict__ coff, int64_t c, int32_t g) {
+    ChunkAhead a;
+    a.g = g;
+    a.task = task;
+    const RowSpan s = row_span(T, g);
+    a.xs = s.xs;
+    a.ys = s.ys;
+    a.lx = s.lx;
+    a.ly = s.ly;
+    a.cs = as_global(cstart)[g];
+    a.out = kWrite ? as_global(coff)[c] : 0;
+    return a;
+}
 
 // One wave per chunk of kChunk merged positions of a row.  The chunk start
 // is split once in global memory (merge path, ties to X); the wave then walks
@@ -137,11 +159,14 @@ constexpr int32_t kGuardHi = INT32_MAX;   // "no Y entry after the row"
 // is read on the device (cstart[rows]); grid-stride so the launch can be
 // replayed from a graph.
 template <bool kWrite>
-__global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__ tasks, int nt, int32_t rows,
+__global__ __launch_bounds__(256, kWrite ? 6 : 8) void k_add_pass(const AddTaskDev *__restrict__ tasks, int nt, int32_t rows,
                                                   const int64_t *__restrict__ cstart,
                                                   const int32_t *__restrict__ crow, int32_t *__restrict__ ccount,
                                                   const int64_t *__restrict__ coff, int32_t *__restrict__ oi,
                                                   double *__restrict__ ov) {
+    constexpr int kSub = kWrite ? 256 : 512;          // merged positions per LDS-staged sub-step
+    constexpr int kPerLane = kSub / kWave;            // merged positions per lane per sub-step
+    constexpr int kStage = (kSub + 1) / kWave + 1;    // tile entries (kSub + 1) per lane, rounded up
     constexpr int kVW = kWrite ? 4 : 1;
     constexpr int kVN = kWrite ? kSub + 2 : 1;
     __shared__ int32_t s_xi[4][kSub + 2], s_yi[4][kSub + 2];
@@ -150,21 +175,59 @@ __global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__
     int32_t *XI = s_xi[w], *YI = s_yi[w];
     double *XV = s_xv[kWrite ? w : 0], *YV = s_yv[kWrite ? w : 0];
     const int64_t nchunks = cstart[rows];
-    for (int64_t c = (int64_t)blockIdx.x * 4 + w; c < nchunks; c += (int64_t)gridDim.x * 4) {
-        const int32_t g = __builtin_amdgcn_readfirstlane(crow[c]);
-        const AddTaskDev &T = tasks[add_task_of(tasks, nt, g)];
-        const RowSpan s = row_span(T, g);
+    const int64_t stride = (int64_t)gridDim.x * 4;
+    int64_t c = (int64_t)blockIdx.x * 4 + w;
+    // A chunk's setup (row -> task -> row spans, chunk start, output offset)
+    // is a chain of dependent loads.  It runs one chunk ahead, the row two
+    // ahead, in lane 0 with vector loads (they count on vmcnt; scalar loads
+    // would share the LDS counter and be waited for by the merge's first
+    // LDS wait).  The task search reads the small task table through the
+    // scalar cache.
+    ChunkAhead nx = {};
+    int32_t g_after = 0;  // lane 0: row of the chunk after the next one
+    if (c < nchunks) {
+        const int32_t g = crow[c];
+        const int ti = add_task_of(tasks, nt, g);
+        const AddTaskDev &T = tasks[ti];
+        if (l == 0) nx = chunk_ahead<kWrite>(T, ti, cstart, coff, c, g);
+    }
+    if (l == 0 && c + stride < nchunks) g_after = as_global(crow)[c + stride];
+    for (; c < nchunks; c += stride) {
+        const ChunkAhead cur = nx;
+        const int32_t g = __builtin_amdgcn_readfirstlane(cur.g);
+        const int ti = __builtin_amdgcn_readfirstlane(cur.task);
+        const AddTaskDev &T = tasks[ti];
+        RowSpan s;
+        s.xs = ((int64_t)__builtin_amdgcn_readfirstlane((int32_t)(cur.xs >> 32)) << 32) |
+               (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)cur.xs);
+        s.ys = ((int64_t)__builtin_amdgcn_readfirstlane((int32_t)(cur.ys >> 32)) << 32) |
+               (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)cur.ys);
+        s.lx = __builtin_amdgcn_readfirstlane(cur.lx);
+        s.ly = __builtin_amdgcn_readfirstlane(cur.ly);
+        const int64_t cs = ((int64_t)__builtin_amdgcn_readfirstlane((int32_t)(cur.cs >> 32)) << 32) |
+                           (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)cur.cs);
+        int64_t out = kWrite ? (((int64_t)__builtin_amdgcn_readfirstlane((int32_t)(cur.out >> 32)) << 32) |
+                                (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)cur.out))
+                             : 0;
+        // the next chunk's setup and the row of the one after it
+        const int64_t cn = c + stride;
+        if (cn < nchunks) {
+            const int32_t gn = __builtin_amdgcn_readfirstlane(g_after);
+            const int tn = add_task_of(tasks, nt, gn);
+            const AddTaskDev &TN = tasks[tn];
+            if (l == 0) nx = chunk_ahead<kWrite>(TN, tn, cstart, coff, cn, gn);
+            if (l == 0 && cn + stride < nchunks) g_after = as_global(crow)[cn + stride];
+        }
         const TR_GLOBAL int32_t *__restrict__ xi = as_global(T.xi) + s.xs;
         const TR_GLOBAL double *__restrict__ xv = as_global(T.xv) + s.xs;
         const TR_GLOBAL int32_t *__restrict__ yi = as_global(T.yi) + s.ys;
         const TR_GLOBAL double *__restrict__ yv = as_global(T.yv) + s.ys;
-        const int32_t t0 = (int32_t)(c - cstart[g]) * kChunk;
+        const int32_t t0 = (int32_t)(c - cs) * kChunk;
         const int32_t t1 = min(t0 + kChunk, s.lx + s.ly);
         int32_t sp = 0;
-        if (l == 0) sp = merge_split(xi, s.lx, yi, s.ly, t0);
+        if (l == 0 && t0 > 0) sp = merge_split(xi, s.lx, yi, s.ly, t0);
         int32_t i0 = __shfl(sp, 0, kWave);
         int32_t j0 = t0 - i0;
-        int64_t out = kWrite ? coff[c] : 0;
         int32_t kept_total = 0;
         for (int32_t tb = t0; tb < t1; tb += kSub) {
             const int32_t n = min(kSub, t1 - tb);
@@ -245,17 +308,25 @@ __global__ __launch_bounds__(256) void k_add_pass(const AddTaskDev *__restrict__
             }
             const int nkeep = __popc(keep);
             const int32_t incl = wave_incl_scan((int32_t)nkeep);
+            const int32_t kept = __shfl(incl, kWave - 1, kWave);
             if (kWrite) {
-                int64_t q = out + (incl - nkeep);
+                // kept entries staged at their wave-local offsets, then written
+                // with consecutive lanes on consecutive entries
+                wave_sync();
+                int32_t q = incl - nkeep;
 #pragma unroll
                 for (int u = 0; u < kPerLane; ++u)
                     if (keep & (1u << u)) {
-                        oi[q] = keys[u];
-                        ov[q] = vals[u];
+                        XI[q] = keys[u];
+                        XV[q] = vals[u];
                         ++q;
                     }
+                wave_sync();
+                for (int32_t e = l; e < kept; e += kWave) {
+                    __builtin_nontemporal_store(XI[e], &oi[out + e]);
+                    __builtin_nontemporal_store(XV[e], &ov[out + e]);
+                }
             }
-            const int32_t kept = __shfl(incl, kWave - 1, kWave);
             out += kept;
             kept_total += kept;
             const int32_t xs = wave_sum(xcnt);
