@@ -150,6 +150,95 @@ __device__ __forceinline__ void stream_cols_v4(const TR_GLOBAL int32_t *__restri
     if (!kEager && ta + l < e) f(bi[ta + l]);
 }
 
+// Streaming of B rows with their loads one round trip ahead, for kernels
+// whose LDS leaves registers to spare (the 2^20-bit heavy symbolic runs one
+// 16-wave workgroup per CU): each row's first round trip (its unaligned head
+// and tail and its first two 16-byte groups per lane, as stream_cols_v4<true>)
+// is issued while the previous row is applied, and a long row's next 1024
+// columns while the current ones are.
+typedef int32_t i32x4_t __attribute__((ext_vector_type(4)));
+
+struct RowHead {
+    int32_t s, e, sa, nv, jh, jt;
+    i32x4_t g0, g1;
+};
+
+// Loads below are issued unconditionally so that the compiler can keep the
+// next round trip's loads in flight while it waits for the current ones (a
+// branch around a load makes it wait for every load): a lane with nothing to
+// read loads the aligned 16 bytes around an element known to exist (the
+// row's first, or bi[0], which a heavy row's operand always has), which
+// stays on a mapped page, and its value is dropped.
+__device__ __forceinline__ const TR_GLOBAL i32x4_t *safe_block(const TR_GLOBAL int32_t *__restrict__ bi, int32_t s,
+                                                             int32_t e) {
+    return (const TR_GLOBAL i32x4_t *)((uintptr_t)(bi + (e > s ? s : 0)) & ~(uintptr_t)15);
+}
+
+__device__ __forceinline__ RowHead row_head_issue(const TR_GLOBAL int32_t *__restrict__ bi, int32_t s, int32_t e) {
+    RowHead h;
+    const int l = lane_id();
+    const int32_t mis = (int32_t)(((uintptr_t)(bi + s) >> 2) & 3);
+    h.s = s;
+    h.e = e;
+    h.sa = min(e, s + ((4 - mis) & 3));
+    h.nv = (e - h.sa) >> 2;
+    const int32_t ta = h.sa + (h.nv << 2);
+    const TR_GLOBAL i32x4_t *__restrict__ b4 = (const TR_GLOBAL i32x4_t *)(bi + h.sa);
+    const TR_GLOBAL i32x4_t *__restrict__ sb = safe_block(bi, s, e);
+    const TR_GLOBAL int32_t *__restrict__ s1 = (const TR_GLOBAL int32_t *)sb;
+    h.jh = *(s + l < h.sa ? bi + s + l : s1);
+    h.jt = *(ta + l < e ? bi + ta + l : s1);
+    h.g0 = *(l < h.nv ? b4 + l : sb);
+    h.g1 = *(l + kWave < h.nv ? b4 + l + kWave : sb);
+    return h;
+}
+
+template <typename F>
+__device__ __forceinline__ void row_stream_apply(const TR_GLOBAL int32_t *__restrict__ bi, const RowHead &h, F &f) {
+    const int l = lane_id();
+    const int32_t ta = h.sa + (h.nv << 2);
+    if (h.s + l < h.sa) f(h.jh);
+    if (ta + l < h.e) f(h.jt);
+    if (l < h.nv) f(h.g0.x), f(h.g0.y), f(h.g0.z), f(h.g0.w);
+    if (l + kWave < h.nv) f(h.g1.x), f(h.g1.y), f(h.g1.z), f(h.g1.w);
+    if (h.nv <= 2 * kWave) return;  // wave-uniform
+    const TR_GLOBAL i32x4_t *__restrict__ b4 = (const TR_GLOBAL i32x4_t *)(bi + h.sa);
+    const TR_GLOBAL i32x4_t *__restrict__ sb = safe_block(bi, h.s, h.e);
+    i32x4_t c[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int32_t v = 2 * kWave + k * kWave + l;
+        c[k] = *(v < h.nv ? b4 + v : sb);
+    }
+    for (int32_t base = 2 * kWave; base < h.nv; base += 4 * kWave) {  // wave-uniform
+        i32x4_t n[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int32_t v = base + 4 * kWave + k * kWave + l;
+            n[k] = *(v < h.nv ? b4 + v : sb);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (base + k * kWave + l < h.nv) f(c[k].x), f(c[k].y), f(c[k].z), f(c[k].w);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] = n[k];
+    }
+}
+
+// the nk rows [bs, be) held by lanes 0..nk-1, each streamed over the wave
+template <typename F>
+__device__ __forceinline__ void stream_rows_ahead(const TR_GLOBAL int32_t *__restrict__ bi, int32_t bs, int32_t be,
+                                                  int nk, F &&f) {
+    if (nk <= 0) return;
+    RowHead cur = row_head_issue(bi, __shfl(bs, 0, kWave), __shfl(be, 0, kWave));
+    for (int t = 0; t < nk; ++t) {
+        RowHead nxt = cur;
+        if (t + 1 < nk) nxt = row_head_issue(bi, __shfl(bs, t + 1, kWave), __shfl(be, t + 1, kWave));
+        row_stream_apply(bi, cur, f);
+        cur = nxt;
+    }
+}
+
 // Same with the values (all entries of one B row have distinct columns, so
 // the order in which one lane applies them does not matter).
 template <typename F>
@@ -353,6 +442,14 @@ __global__ __launch_bounds__(256) void k_sym_light(const TaskDev *__restrict__ t
     }
 }
 
+// LDS word of presence-bitmap word wd: with kSwz, its bank bits (the low
+// five) XORed with the next two groups of five, a bijection on each aligned
+// group of 32 words (clearing whole groups clears every word in use)
+template <bool kSwz>
+__device__ __forceinline__ int32_t bm_word(int32_t wd) {
+    return kSwz ? wd ^ (((wd >> 5) ^ (wd >> 10)) & 31) : wd;
+}
+
 // ---------------------------------------------------------------- symbolic, heavy rows
 // One NTH-thread block per row (grid-stride over the heavy list): a bitmap
 // of 2^PB columns in LDS per pass, popcounts per 128-column fine block, and
@@ -376,6 +473,12 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
     // limits the kernel to four waves per SIMD (2^20 bits), so the extra
     // registers cost no occupancy
     constexpr bool kEagerB = PB == 20 || (PB == 19 && NTH == 512);
+    // R-MAT column bits are mostly 0, so a quarter of the marks would share a
+    // bank; the 2^20-bit kernel XORs the word's bank bits with higher ones
+    // (R-MAT s20: 34.8 -> 33.2 ms).  In the 64-VGPR kernels the extra work
+    // costs more than it saves (2^19 bits: 16.0 -> 18.4 ms).
+    constexpr bool kSwizzle = PB == 20;
+    static_assert(kWords % 32 == 0, "the bitmap holds whole 32-word swizzle groups");
     __shared__ uint32_t bm[kWords];
     __shared__ int32_t wsum[kWavesB];
     __shared__ int32_t sh_wbase;
@@ -400,7 +503,7 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
             const int32_t nfine = (width + (1 << kFineShift) - 1) >> kFineShift;
             // only the pass's words are marked and counted (a narrow matrix
             // leaves the rest of the bitmap unused)
-            for (int t = tid; t < nwords; t += NTH) bm[t] = 0u;
+            for (int t = tid; t < (kSwizzle ? ((nwords + 31) & ~31) : nwords); t += NTH) bm[t] = 0u;
             __syncthreads();
             // mark: entry pb + w + 16 t goes to wave w
             for (int32_t pb = a0; pb < a1; pb += kWavesB * kWave) {
@@ -413,12 +516,17 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
                 }
                 const int32_t rem = a1 - (pb + w);
                 const int nk = rem <= 0 ? 0 : min(kWave, (rem + kWavesB - 1) / kWavesB);
-                for (int t = 0; t < nk; ++t) {
-                    const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
-                    stream_cols_v4<kEagerB>(bi, s, e, [&](int32_t j) {
-                        const int64_t jj = (int64_t)j - base;
-                        if (jj >= 0 && jj < width) atomicOr(&bm[jj >> 5], 1u << (jj & 31));
-                    });
+                auto mark = [&](int32_t j) {
+                    const int64_t jj = (int64_t)j - base;
+                    if (jj >= 0 && jj < width) atomicOr(&bm[bm_word<kSwizzle>((int32_t)(jj >> 5))], 1u << (jj & 31));
+                };
+                if constexpr (kEagerB) {
+                    stream_rows_ahead(bi, bs, be, nk, mark);
+                } else {
+                    for (int t = 0; t < nk; ++t) {
+                        const int32_t s = __shfl(bs, t, kWave), e = __shfl(be, t, kWave);
+                        stream_cols_v4<false>(bi, s, e, mark);
+                    }
                 }
             }
             __syncthreads();
@@ -433,7 +541,7 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int wd = f * 4 + q;
-                        if (wd < nwords) v += __popc(bm[wd]);
+                        if (wd < nwords) v += __popc(bm[bm_word<kSwizzle>(wd)]);
                     }
                 }
                 c[u] = v;
