@@ -329,6 +329,24 @@ def test_transpose_dense():
     assert oracle.rows_sorted(T)
 
 
+def test_rows_sorted_rejects():
+    # strictly increasing columns within each row, rows independent: a swap,
+    # a repeated column or a descent anywhere in any row makes it false, and
+    # a descent across a row boundary does not
+    ok = gen.from_arrays(3, 9, [0, 3, 3, 6], [1, 4, 8, 0, 2, 5], np.ones(6))
+    assert oracle.rows_sorted(ok)
+    for idx in ([4, 1, 8, 0, 2, 5], [1, 4, 4, 0, 2, 5], [1, 4, 8, 0, 5, 2], [1, 4, 8, 2, 2, 5]):
+        assert not oracle.rows_sorted(gen.from_arrays(3, 9, [0, 3, 3, 6], idx, np.ones(6))), idx
+    assert oracle.rows_sorted(gen.from_arrays(2, 9, [0, 2, 4], [7, 8, 0, 1], np.ones(4)))
+    # one descent deep in a large random matrix
+    R = gen.rmat(10, 8, seed=4)
+    idx = R.idx.copy()
+    r = int(np.argmax(np.diff(R.ptr) >= 3))
+    p = int(R.ptr[r]) + 1
+    idx[p], idx[p + 1] = idx[p + 1], idx[p]
+    assert oracle.rows_sorted(R) and not oracle.rows_sorted(gen.from_arrays(R.nrows, R.ncols, R.ptr, idx, R.val))
+
+
 # ---------------------------------------------------------------- generators
 def test_generators_shapes_and_sortedness():
     A = gen.erdos_renyi(1024, 8, seed=0)
