@@ -468,7 +468,7 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
     constexpr int kFine = 1 << (PB - kFineShift);  // fine blocks of one pass
     constexpr int kPerThread = kFine / NTH;        // consecutive fine blocks per thread: 2..8
     constexpr int kWavesB = NTH / kWave;
-    static_assert(kPerThread >= 1 && kPerThread <= 8, "fine blocks per thread");
+    static_assert(kPerThread >= 1 && kPerThread <= 8 && (kPerThread & (kPerThread - 1)) == 0, "fine blocks per thread");
     // B rows are streamed with eager loads where the bitmap's LDS already
     // limits the kernel to four waves per SIMD (2^20 bits), so the extra
     // registers cost no occupancy
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
     // costs more than it saves (2^19 bits: 16.0 -> 18.4 ms).
     constexpr bool kSwizzle = PB == 20;
     static_assert(kWords % 32 == 0, "the bitmap holds whole 32-word swizzle groups");
-    __shared__ uint32_t bm[kWords];
+    __shared__ __align__(16) uint32_t bm[kWords];
     __shared__ int32_t wsum[kWavesB];
     __shared__ int32_t sh_wbase;
 
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
             const int32_t nfine = (width + (1 << kFineShift) - 1) >> kFineShift;
             // only the pass's words are marked and counted (a narrow matrix
             // leaves the rest of the bitmap unused)
-            for (int t = tid; t < (kSwizzle ? ((nwords + 31) & ~31) : nwords); t += NTH) bm[t] = 0u;
+            for (int t = tid; t < (kSwizzle ? ((nwords + 31) & ~31) : ((nwords + 3) & ~3)); t += NTH) bm[t] = 0u;
             __syncthreads();
             // mark: entry pb + w + 16 t goes to wave w
             for (int32_t pb = a0; pb < a1; pb += kWavesB * kWave) {
@@ -530,21 +530,28 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
                 }
             }
             __syncthreads();
-            // fine-block counts, kPerThread consecutive fine blocks per thread
+            // fine-block counts, kPerThread consecutive fine blocks per thread.
+            // A fine block's four words are one 16-byte read (the swizzle
+            // keeps them in their aligned 16 bytes), and read k of lane l
+            // takes block (k + l / (8 / kPerThread)) mod kPerThread: eight
+            // consecutive lanes then read eight different 16-byte bank groups
+            // (in order, lanes kPerThread blocks apart would share one or two:
+            // a 32- or 64-way conflict on every read)
             int32_t c[kPerThread];
             int32_t tsum = 0;
 #pragma unroll
-            for (int u = 0; u < kPerThread; ++u) {
+            for (int u = 0; u < kPerThread; ++u) c[u] = 0;
+#pragma unroll
+            for (int k = 0; k < kPerThread; ++k) {
+                const int u = (k + l / (8 / kPerThread)) & (kPerThread - 1);
                 const int f = tid * kPerThread + u;
                 int32_t v = 0;
                 if (f < nfine) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int wd = f * 4 + q;
-                        if (wd < nwords) v += __popc(bm[bm_word<kSwizzle>(wd)]);
-                    }
+                    const uint4 x = *reinterpret_cast<const uint4 *>(&bm[bm_word<kSwizzle>(f * 4) & ~3]);
+                    v = __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
                 }
-                c[u] = v;
+#pragma unroll
+                for (int uu = 0; uu < kPerThread; ++uu) c[uu] = uu == u ? v : c[uu];
                 tsum += v;
             }
             const int32_t inc = wave_incl_scan(tsum);
@@ -584,8 +591,16 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
                 __syncthreads();
                 continue;
             }
+            // 16-byte writes: a thread's offsets are consecutive words
+            if constexpr (kPerThread % 4 == 0) {
 #pragma unroll
-            for (int u = 0; u < kPerThread; ++u) fex_lds[tid * kPerThread + u] = fex[u];
+                for (int u = 0; u < kPerThread; u += 4)
+                    *reinterpret_cast<int4 *>(&fex_lds[tid * kPerThread + u]) =
+                        make_int4(fex[u], fex[u + 1], fex[u + 2], fex[u + 3]);
+            } else {
+#pragma unroll
+                for (int u = 0; u < kPerThread; ++u) fex_lds[tid * kPerThread + u] = fex[u];
+            }
             __syncthreads();
             // window starts: a count quantum or a width quantum begins.  A
             // sparse pass (few entries per width quantum) is cut by a smaller
@@ -598,8 +613,10 @@ __global__ __launch_bounds__(NTH, (PB <= 18 || (PB == 19 && NTH == 1024)) ? 8 : 
             for (int u = 0; u < kPerThread; ++u) {
                 const int f = tid * kPerThread + u;
                 bool first = false;
+                // the previous block's offset is this thread's, except for its first block
+                const int32_t prev = u == 0 ? (f > 0 ? fex_lds[f - 1] : 0) : fex[u - 1];
                 if (f < nfine)
-                    first = (f == 0) || (fex[u] / q != fex_lds[f - 1] / q) ||
+                    first = (f == 0) || (fex[u] / q != prev / q) ||
                             (!sparse_pass && ((f >> kWidthShift) != ((f - 1) >> kWidthShift)));
                 if (first) {
                     flags |= 1 << u;
