@@ -873,10 +873,37 @@ __global__ __launch_bounds__(WPB * kWave, T <= 256 ? 8 : 1) void k_num_light(con
                     for (int32_t x0 = 0; x0 < total; x0 += kWave) {
                         const int32_t x = x0 + l;
                         int t = 0;
+                        int32_t q;
+                        if constexpr (kMask && T >= 2 * kWave) {
+                            // The lanes whose B rows meet this round write (lane,
+                            // B offset) at the position where their rows start in
+                            // it (or 0) and mark it in a start mask; position x
+                            // belongs to the highest start at or below it.  The
+                            // first 65 lane-mask words, all zero between rounds,
+                            // hold both and are cleared again before the round
+                            // uses them.  Three round trips instead of the six
+                            // dependent shuffles of a search over the lanes.
+                            const bool meets = len > 0 && excl < x0 + kWave && excl + len > x0;
+                            const int32_t pos = max(excl - x0, 0);
+                            if (meets) {
+                                own[pos] = ((unsigned long long)(uint32_t)l << 32) | (uint32_t)(bs - excl);
+                                atomicOr(&own[kWave], 1ull << pos);
+                            }
+                            wave_sync();
+                            const unsigned long long starts = own[kWave] & (~0ull >> (63 - l));
+                            const unsigned long long e = own[63 - __clzll((long long)starts)];
+                            t = (int)(e >> 32);
+                            q = (int32_t)(uint32_t)e + x;
+                            wave_sync();
+                            own[l] = 0ull;
+                            if (l == 0) own[kWave] = 0ull;
+                            wave_sync();
+                        } else {
 #pragma unroll
-                        for (int step = kWave / 2; step > 0; step >>= 1)
-                            if (__shfl(excl, t + step, kWave) <= x) t += step;
-                        const int32_t q = __shfl(bs, t, kWave) + (x - __shfl(excl, t, kWave));
+                            for (int step = kWave / 2; step > 0; step >>= 1)
+                                if (__shfl(excl, t + step, kWave) <= x) t += step;
+                            q = __shfl(bs, t, kWave) + (x - __shfl(excl, t, kWave));
+                        }
                         const double at = __shfl(a, t, kWave);
                         bool pend = x < total;
                         uint32_t h = 0;
