@@ -272,6 +272,43 @@ def test_heavy_ordered_collisions(ctx):
     assert frac > 0.5
 
 
+def test_degenerate_shapes(ctx):
+    # zero rows, an empty inner dimension, rows with nothing, 1 x 1
+    rng = np.random.default_rng(31)
+    B = gen.rmat(8, 8, seed=5)
+    A0 = gen.from_arrays(0, B.nrows, [0], [], [])
+    C = run(ctx, A0, B, "0-row A")
+    assert C.nrows == 0 and C.nnz == 0
+    Ak = gen.from_arrays(5, 0, np.zeros(6), [], [])
+    Bk = gen.from_arrays(0, 7, [0], [], [])
+    C = run(ctx, Ak, Bk, "empty inner dimension")
+    assert (C.nrows, C.ncols, C.nnz) == (5, 7, 0)
+    ptr = np.zeros(B.nrows + 1, np.int64)
+    ptr[101:] = 3
+    Ae = gen.from_arrays(B.nrows, B.nrows, ptr, [3, 7, 200], rng.uniform(-1, 1, 3))  # one non-empty row
+    run(ctx, Ae, B, "all rows empty but one")
+    run(ctx, gen.from_arrays(1, 1, [0, 1], [0], [3.0]), gen.from_arrays(1, 1, [0, 1], [0], [-0.5]), "1 x 1")
+
+
+@pytest.mark.parametrize("per_row", [40, 300, 3000], ids=["light", "mid", "heavy"])
+def test_single_column_chains(ctx, per_row):
+    # every B row holds only column 0, so each C row is one entry: the fma
+    # chain of all its products in A-order (per_row of them, values over
+    # 2^-20..2^20), taken by the light path, the wave-flattened short rows
+    # and the compact heavy rows
+    rng = np.random.default_rng(per_row)
+    k = 4000
+    B = gen.from_arrays(k, 1, np.arange(k + 1), np.zeros(k, np.int32),
+                        rng.uniform(-1, 1, k) * np.exp2(rng.integers(-20, 21, k)))
+    m = 64
+    al = [np.sort(rng.choice(k, size=per_row, replace=False)) for _ in range(m)]
+    aptr = np.concatenate([[0], np.cumsum([len(x) for x in al])])
+    aval = rng.uniform(-1, 1, aptr[-1]) * np.exp2(rng.integers(-20, 21, aptr[-1]))
+    A = gen.from_arrays(m, k, aptr, np.concatenate(al), aval)
+    C = run(ctx, A, B, f"single column, {per_row} products per entry")
+    assert C.nnz == m
+
+
 def _device_bytes_equal(ctx, pa, pb, nbytes, chunk=1 << 29):
     """Byte equality of two device ranges, copied to the host in chunks."""
     ha = np.empty(max(1, min(chunk, nbytes)), np.uint8)
