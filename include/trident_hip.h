@@ -166,7 +166,11 @@ int tr_spgemm_flops(tr_ctx_t ctx, const tr_csx_t *A, const tr_csx_t *B, int64_t 
  *  - item_windows: heavy windows per work item, [1, 64] (rows of more than
  *    2048 entries use a quarter of it, at least 1);
  *  - heavy_table: slots of the heavy numeric table, 2048 (256-thread
- *    workgroups) or 4096 (512-thread workgroups).
+ *    workgroups) or 4096 (512-thread workgroups);
+ *  - dense_width: heavy windows spanning at most this many columns use a
+ *    dense accumulator (slot = column offset: no hashing or probing, and
+ *    the slots in order are the sorted output), [0, heavy_table]; 0 hashes
+ *    every window.
  * Results do not depend on the tuning (bit-identical), only the time does.
  * tr_tuning_default gives the built-in values; tr_set_default_tuning sets
  * what contexts created afterwards start with (server workers included;
@@ -177,6 +181,7 @@ typedef struct tr_tuning {
     int32_t window_entries;
     int32_t item_windows;
     int32_t heavy_table;
+    int32_t dense_width;
 } tr_tuning_t;
 int tr_tuning_default(tr_tuning_t *t);
 int tr_set_default_tuning(const tr_tuning_t *t);

@@ -57,8 +57,10 @@ def test_tuning_defaults_and_validation_without_gpu():
     lib = _lib.load()
     t = _lib.tr_tuning_t()
     assert lib.tr_tuning_default(ctypes.byref(t)) == _lib.TR_OK
-    assert (t.light_max_flops, t.window_entries, t.item_windows, t.heavy_table) == (1024, 1152, 16, 2048)
+    assert (t.light_max_flops, t.window_entries, t.item_windows, t.heavy_table, t.dense_width) == \
+        (1024, 1152, 16, 2048, 2048)
     bad = [dict(light_max_flops=2048), dict(light_max_flops=32), dict(heavy_table=3000),
+           dict(dense_width=-1), dict(dense_width=2049),
            dict(window_entries=1410), dict(window_entries=32), dict(item_windows=0), dict(item_windows=65)]
     for b in bad:
         u = _lib.tr_tuning_t()
@@ -68,8 +70,11 @@ def test_tuning_defaults_and_validation_without_gpu():
         assert lib.tr_set_default_tuning(ctypes.byref(u)) == _lib.TR_ERR_INVALID, b
     u = _lib.tr_tuning_t()
     lib.tr_tuning_default(ctypes.byref(u))
-    u.heavy_table, u.window_entries = 4096, 1920
+    u.heavy_table, u.window_entries, u.dense_width = 4096, 1920, 4096
     assert lib.tr_set_default_tuning(ctypes.byref(u)) == _lib.TR_OK
+    u.dense_width = 4097
+    assert lib.tr_set_default_tuning(ctypes.byref(u)) == _lib.TR_ERR_INVALID
+    u.dense_width = 0
     u.heavy_table, u.window_entries = 2048, 1409  # 3/4 of the table - 127
     assert lib.tr_set_default_tuning(ctypes.byref(u)) == _lib.TR_OK
     assert lib.tr_set_default_tuning(None) == _lib.TR_OK

@@ -59,7 +59,10 @@ def test_rmat(ctx, scale, ef, seed):
 
 TUNINGS = [
     dict(heavy_table=4096, window_entries=1920),
+    dict(heavy_table=4096, window_entries=1920, dense_width=4096),
     dict(heavy_table=4096, window_entries=64, item_windows=1),
+    dict(dense_width=0),
+    dict(dense_width=512, window_entries=1409),
     dict(window_entries=64, item_windows=64, light_max_flops=64),
     dict(window_entries=384, item_windows=4, light_max_flops=256),
     dict(window_entries=1409, item_windows=2),
@@ -270,6 +273,39 @@ def test_heavy_ordered_collisions(ctx):
     frac = float(np.mean(rv[multi] != C.val[multi]))
     print(f"reverse A-order changes {frac:.1%} of the hub and semi-hub entries", flush=True)
     assert frac > 0.5
+
+
+@pytest.mark.parametrize("tuning", [dict(dense_width=0), dict(dense_width=2048),
+                                    dict(heavy_table=4096, window_entries=2816, dense_width=4096)],
+                         ids=["hashed", "dense2048", "dense4096"])
+def test_dense_accumulator_windows(ctx, tuning):
+    # Heavy rows whose output is dense over a few thousand columns: windows
+    # of about 1150 (or 2800) entries span fewer columns than the table has
+    # slots, so with dense_width set they take the direct-addressed
+    # accumulator; every eighth B row adds a column far out, so each row also
+    # has a wide sparse window that keeps hashing.  Hub columns 0..3 sit in
+    # every B row (long slot lists in every step).  Hashed and direct runs
+    # must both match the oracle bit for bit.
+    rng = np.random.default_rng(41)
+    k, ncols, m = 2500, 60000, 40
+    bl = []
+    for i in range(k):
+        cols = np.concatenate([np.arange(4), rng.choice(np.arange(4, 3000), size=36, replace=False)])
+        if i % 8 == 0:
+            cols = np.concatenate([cols, [rng.integers(3000, ncols)]])
+        bl.append(np.sort(cols))
+    bptr = np.concatenate([[0], np.cumsum([len(x) for x in bl])])
+    bval = rng.uniform(-1, 1, bptr[-1]) * np.exp2(rng.integers(-20, 21, bptr[-1]))
+    B = gen.from_arrays(k, ncols, bptr, np.concatenate(bl), bval)
+    al = [np.sort(rng.choice(k, size=int(rng.integers(300, 1200)), replace=False)) for _ in range(m)]
+    aptr = np.concatenate([[0], np.cumsum([len(x) for x in al])])
+    aval = rng.uniform(-1, 1, aptr[-1]) * np.exp2(rng.integers(-20, 21, aptr[-1]))
+    A = gen.from_arrays(m, k, aptr, np.concatenate(al), aval)
+    ctx.set_tuning(dict(ctx.get_tuning(), **tuning))
+    try:
+        run(ctx, A, B, f"dense accumulator {tuning}")
+    finally:
+        ctx.set_tuning(None)
 
 
 def test_degenerate_shapes(ctx):

@@ -54,6 +54,7 @@ class tr_tuning_t(ctypes.Structure):
         ("window_entries", ctypes.c_int32),
         ("item_windows", ctypes.c_int32),
         ("heavy_table", ctypes.c_int32),
+        ("dense_width", ctypes.c_int32),
     ]
 
 

@@ -1,17 +1,18 @@
 """Autotuner of the multiply cutovers (tr_tuning_t) on the GPU.
 
-The multiply has four cutovers, and none of them changes a result bit:
+The multiply has five cutovers, and none of them changes a result bit:
 
 - the flop count above which a row takes the heavy (column-windowed) path;
 - the window packing quantum H;
 - the number of windows per heavy work item;
-- the heavy table (2048 slots on 256 threads, or 4096 slots on 512).
+- the heavy table (2048 slots on 256 threads, or 4096 slots on 512);
+- the widest heavy window (in columns) that takes the dense accumulator.
 
 This tool times the workload's multiply under candidate settings and keeps
 the fastest. The search is coordinate descent from the built-in values:
 
 - the heavy table and its window quantum are searched together, then the
-  items, then the light/heavy cut;
+  items, the dense width, then the light/heavy cut;
 - a second sweep runs when the first one moved anything.
 
 It writes a JSON document whose "tuning" object `device.load_tuning`
@@ -54,6 +55,7 @@ TUNING_DIR = os.path.join(HERE, "tuning")
 TABLE_WINDOWS = [(2048, 384), (2048, 640), (2048, 896), (2048, 1024), (2048, 1152), (2048, 1280), (2048, 1408), (4096, 1920), (4096, 2816)]
 ITEM_WINDOWS = [1, 2, 4, 8, 16, 32]
 LIGHT_MAX = [256, 512, 1024]
+DENSE_FRACTION = [0, 4, 2, 1]  # dense_width = heavy_table / d (0: none)
 
 
 def default_path(workload: str) -> str:
@@ -188,12 +190,18 @@ def tune(workload: str, reps: int = 2, log=print) -> dict:
     for sweep in range(2):
         moved = False
         for table, h in TABLE_WINDOWS:
-            cand = dict(best, heavy_table=table, window_entries=h)
+            cand = dict(best, heavy_table=table, window_entries=h,
+                        dense_width=min(best["dense_width"], table))
             t = measure(cand)
             if t < best_t:
                 best, best_t, moved = cand, t, True
         for iw in ITEM_WINDOWS:
             cand = dict(best, item_windows=iw)
+            t = measure(cand)
+            if t < best_t:
+                best, best_t, moved = cand, t, True
+        for d in DENSE_FRACTION:
+            cand = dict(best, dense_width=best["heavy_table"] // d if d else 0)
             t = measure(cand)
             if t < best_t:
                 best, best_t, moved = cand, t, True

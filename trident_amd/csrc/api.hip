@@ -282,6 +282,7 @@ tr_tuning_t tuning_builtin() {
     t.window_entries = 1152;
     t.item_windows = 16;
     t.heavy_table = 2048;
+    t.dense_width = 2048;
     return t;
 }
 
@@ -302,6 +303,7 @@ void tuning_check(const tr_tuning_t &t) {
     TR_REQUIRE(t.window_entries >= 64 && t.window_entries <= 3 * t.heavy_table / 4 - 127,
                "tuning: window_entries must be in [64, 3 * heavy_table / 4 - 127]");
     TR_REQUIRE(t.item_windows >= 1 && t.item_windows <= 64, "tuning: item_windows must be in [1, 64]");
+    TR_REQUIRE(t.dense_width >= 0 && t.dense_width <= t.heavy_table, "tuning: dense_width must be in [0, heavy_table]");
 }
 
 }  // namespace tr

@@ -1162,7 +1162,8 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
                                                      const int32_t *__restrict__ order,
                                                      const int64_t *__restrict__ cursor_base,
                                                      int4 *__restrict__ cursors, const int64_t *__restrict__ rowptr,
-                                                     const Panel *__restrict__ panels, int npanels) {
+                                                     const Panel *__restrict__ panels, int npanels,
+                                                     int32_t dense_width) {
     constexpr uint32_t kLog = __builtin_ctz(T);
     __shared__ HeavyWgLds<T, NT> L;
     const int tid = threadIdx.x, l = lane_id();
@@ -1218,6 +1219,11 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
             // windows hold <= kSparseH + 127 entries and are ranked by
             // counting smaller keys
             const bool dense = (W.c1 - W.c0) <= kWinMaxWidth;
+            // windows no wider than dense_width (<= T) address their slots by
+            // column offset: no probing, no ranking pass.  The slot is
+            // (off * hmul) >> (32 - kLog) either way, with hmul = 2^(32 - kLog)
+            // for the identity and the golden-ratio multiplier for the hash.
+            const uint32_t hmul = (W.c1 - W.c0) <= dense_width ? (1u << (32 - kLog)) : 0x9E3779B1u;
             int nfresh = 0;
             for (int32_t pb = a0, slot = 0; pb < a1; pb += NT, ++slot) {
                 if (tid < 2 * (NT / kWave)) (&L.smask[0][0])[tid] = 0ull;
@@ -1311,7 +1317,7 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
                         const uint32_t off = (uint32_t)(j - W.c0);
                         key = off << kTagBits;
                         me = key | (uint32_t)tid;
-                        h = hash_slot((int32_t)off, kLog);
+                        h = (off * hmul) >> (32 - kLog);
                         old = atomicCAS(&L.keytag[h], kSlotFree, me);
                     }
                     if (xn < total) {
@@ -1443,23 +1449,38 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
             if (dense) {
                 // the thread's slots (h = tid + u NT) by column offset, ranked
                 // one kBmWidth-column pass at a time with a presence bitmap
-                // (it shares LDS with the step arrays, which are dead now)
+                // (it shares LDS with the step arrays, which are dead now).
+                // A direct window's slots are its column offsets, so a ballot
+                // over 64 consecutive slots is its presence word, and its one
+                // pass (width <= T < kBmWidth) needs no clearing or atomics.
+                const int32_t width = W.c1 - W.c0;
+                const bool direct = width <= dense_width;
 #pragma unroll
                 for (int u = 0; u < kSt; ++u) {
                     const int h = tid + u * NT;
                     const uint32_t kt = h < T ? L.keytag[h] : kSlotFree;
                     st_rk[u] = kt == kSlotFree ? kNone : kt >> kTagBits;
                 }
-                const int32_t width = W.c1 - W.c0;
+                if (direct) {
+                    const int wsl = __builtin_amdgcn_readfirstlane(tid / kWave);  // word of slot tid in round 0
+#pragma unroll 1
+                    for (int u = 0; u < kSt; ++u) {
+                        const uint64_t m = ballot(L.keytag[tid + u * NT] != kSlotFree);
+                        if (l == 0) L.bm[u * (NT / kWave) + wsl] = m;
+                    }
+                }
+                static_assert(T / kWave <= kBmWords, "a direct window's slots fit one pass");
                 int32_t base_rank = 0;
                 for (int32_t lo = 0; lo < width; lo += kBmWidth) {  // block-uniform
                     const int nwords = min(kBmWords, (width - lo + 63) >> 6);
-                    for (int t = tid; t < nwords; t += NT) L.bm[t] = 0ull;
-                    __syncthreads();
+                    if (!direct) {
+                        for (int t = tid; t < nwords; t += NT) L.bm[t] = 0ull;
+                        __syncthreads();
 #pragma unroll
-                    for (int u = 0; u < kSt; ++u) {
-                        const uint32_t rel = st_rk[u] - (uint32_t)lo;  // kNone wraps past kBmWidth
-                        if (rel < (uint32_t)kBmWidth) atomicOr(&L.bm[rel >> 6], 1ull << (rel & 63));
+                        for (int u = 0; u < kSt; ++u) {
+                            const uint32_t rel = st_rk[u] - (uint32_t)lo;  // kNone wraps past kBmWidth
+                            if (rel < (uint32_t)kBmWidth) atomicOr(&L.bm[rel >> 6], 1ull << (rel & 63));
+                        }
                     }
                     __syncthreads();
                     // prefix popcounts of the presence words
@@ -1494,6 +1515,8 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
                     base_rank += tot;
                     __syncthreads();  // wsum, bm and wpre are reused by the next pass
                 }
+            }
+            if (dense) {
                 if (tid == 0 && L.nocc != W.cnt) atomicOr(&cnt->overflow, 4);
 #pragma unroll
                 for (int u = 0; u < kStash; ++u) {
@@ -1762,10 +1785,10 @@ void mul_numeric(tr_ctx *ctx, MulPlan &pl, Panel *dp, int np) {
         const uint32_t grid = cap_grid(pl.h.items, 1 << 22);
         if (pl.tune.heavy_table == 4096)
             TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<4096, 512>), grid, 512, 0, pl.d_tasks, pl.nt, pl.items,
-                      pl.cnt, pl.wins, pl.order, pl.cbase, pl.cursors, pl.rowptr, dp, np);
+                      pl.cnt, pl.wins, pl.order, pl.cbase, pl.cursors, pl.rowptr, dp, np, pl.tune.dense_width);
         else
             TR_LAUNCH(ctx, "num_heavy", (k_num_heavy_wg<2048, kSmallNT>), grid, kSmallNT, 0, pl.d_tasks, pl.nt, pl.items,
-                      pl.cnt, pl.wins, pl.order, pl.cbase, pl.cursors, pl.rowptr, dp, np);
+                      pl.cnt, pl.wins, pl.order, pl.cbase, pl.cursors, pl.rowptr, dp, np, pl.tune.dense_width);
     }
 }
 

@@ -23,7 +23,7 @@ def _np_ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
-TUNING_FIELDS = ("light_max_flops", "window_entries", "item_windows", "heavy_table")
+TUNING_FIELDS = ("light_max_flops", "window_entries", "item_windows", "heavy_table", "dense_width")
 
 
 def _tuning_struct(tuning: dict) -> "_lib.tr_tuning_t":
