@@ -1357,9 +1357,8 @@ __global__ __launch_bounds__(NT, (NT <= 256 ? 5 : 4)) void k_num_heavy_wg(const 
                     }
                     __syncthreads();
                     // the last pusher of each slot applies the slot's products of
-                    // this step in thread (= A) order.  The tag, the pusher's link
-                    // and the slot's value are read in one round trip, and each
-                    // list member's operands with its link; lists of up to three
+                    // this step in thread (= A) order.  Each list member's
+                    // operands are read with its link; lists of up to three
                     // are ordered in registers, longer ones walked through a
                     // bitmask of thread ids (reading two or four operands per
                     // round trip there measured 2% and 7% slower)
