@@ -154,8 +154,8 @@ def _cpu_model() -> str:
 
 def cpu_env():
     """Environment of the CPU-baseline child process: OpenMP threads = the
-    physical cores this process may use, bound close (BASELINE.md's
-    setting).  Binding is confined to the child, so it never pins this
+    physical cores this process may use, bound close (OMP_PROC_BIND=close).
+    Binding is confined to the child, so it never pins this
     process's own threads (the HIP runtime's, the server's workers)."""
     phys, logical = _cpu_topology()
     env = dict(os.environ, OMP_NUM_THREADS=str(phys), OMP_PROC_BIND="close")
