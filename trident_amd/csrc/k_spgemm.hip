@@ -1089,27 +1089,30 @@ struct HeavyWgLds {
 };
 
 // marks the segment [excl, excl + len) of the calling thread in the step that
-// starts at product `base`
+// starts at product `base`: at its first position in the step, and at every
+// later position of the step that starts a 64-position word, so that each
+// word's first position names the segment covering it
 template <int NT, typename M>
 __device__ __forceinline__ void mark_segment(unsigned long long *smask, M *smark, int32_t base, int32_t excl,
                                              int32_t len) {
     if (len <= 0) return;
     const int32_t rel = excl - base;
-    int32_t pos = -1;
-    if (rel >= 0 && rel < NT) pos = rel;
-    else if (rel < 0 && excl + len > base) pos = 0;
-    if (pos >= 0) {
-        smark[pos] = (M)threadIdx.x;
-        atomicOr(&smask[pos >> 6], 1ull << (pos & 63));
+    const int32_t lo = rel > 0 ? rel : 0, hi = rel + len < NT ? rel + len : NT;
+    if (lo >= hi) return;  // the segment has no product in this step
+    smark[lo] = (M)threadIdx.x;
+    atomicOr(&smask[lo >> 6], 1ull << (lo & 63));
+    for (int32_t ws = (lo & ~63) + 64; ws < hi; ws += 64) {
+        smark[ws] = (M)threadIdx.x;
+        atomicOr(&smask[ws >> 6], 1ull);
     }
 }
 
-// segment of step position rel: the mark at the highest set bit <= rel
+// segment of step position rel: the mark at the highest set bit <= rel,
+// always inside rel's own word (its first position is marked)
 template <typename M>
 __device__ __forceinline__ int owner_marked(const unsigned long long *smask, const M *smark, int rel) {
-    int w = rel >> 6;
-    unsigned long long m = smask[w] & (~0ull >> (63 - (rel & 63)));
-    while (m == 0ull && w > 0) m = smask[--w];
+    const int w = rel >> 6;
+    const unsigned long long m = smask[w] & (~0ull >> (63 - (rel & 63)));
     return (int)smark[w * 64 + 63 - __clzll((long long)m)];
 }
 
